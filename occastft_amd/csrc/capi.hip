@@ -1,0 +1,530 @@
+// capi.hip -- the C ABI declared in include/occastft_hip.h.
+//
+// Host side of the engine: option parsing with Default_Pipeline's semantics,
+// the reference's float32 frame geometry, resident FP64-generated tables, and
+// the choice of kernel path per request.
+#include "../../include/occastft_hip.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cerrno>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "engine.hpp"
+#include "multipass.hpp"
+
+struct ostft_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;  // internal: table builds and host-buffer calls
+    std::mutex mu;                 // guards the table caches
+    std::map<std::pair<int64_t, int64_t>, float2*> twiddles;  // (m, count) -> W_m^j, j < count
+    std::map<std::tuple<int, int64_t, uint32_t, int>, float*> windows;  // (kind, N, sigma bits, exact)
+    float* d_in = nullptr;
+    uint64_t d_in_cap = 0;
+    float* d_out = nullptr;
+    uint64_t d_out_cap = 0;
+    ostft::Scratch scratch;
+    std::string last_error;
+};
+
+namespace {
+
+// Restores the caller's current device on scope exit.
+struct DeviceGuard {
+    int prev = -1;
+    bool ok = false;
+    explicit DeviceGuard(int dev)
+    {
+        if (hipGetDevice(&prev) != hipSuccess)
+            prev = -1;
+        ok = hipSetDevice(dev) == hipSuccess;
+    }
+    ~DeviceGuard()
+    {
+        if (prev >= 0)
+            (void)hipSetDevice(prev);
+    }
+};
+
+int hip_fail(ostft_ctx* ctx, hipError_t e, const char* where)
+{
+    if (ctx) {
+        ctx->last_error = std::string(where) + ": " + hipGetErrorString(e);
+    }
+    return OSTFT_ERR_HIP;
+}
+
+int fail(ostft_ctx* ctx, int code, const std::string& msg)
+{
+    if (ctx)
+        ctx->last_error = msg;
+    return code;
+}
+
+bool contains(const char* s, const char* key) { return std::strstr(s, key) != nullptr; }
+
+// toQuot, StandAlone/cross_gpgpu/GLOBAL/RunnerInterface.hpp:26-35: float32 throughout.
+#pragma clang fp contract(off)
+double to_quot_f32(float fullSize, float overlapRatio, float windowSize)
+{
+    float q;
+    if (overlapRatio == 0.0f)
+        q = fullSize / windowSize + 1.0f;
+    else
+        q = (fullSize) / (windowSize * (1.0f - overlapRatio)) + 1.0f;
+    return std::trunc((double)q);
+}
+
+uint64_t omove_f32(float windowSize, float overlapRatio)
+{
+    const float mv = windowSize * (1.0f - overlapRatio);
+    return (uint64_t)mv;
+}
+
+struct Plan {
+    ostft_geometry g;
+    int pack;       // ostft::PackMode
+    int log_m;      // complex transform size
+    uint64_t groups;
+};
+
+int make_plan(const ostft_params* p, uint64_t L, Plan* plan, ostft_ctx* ctx)
+{
+    if (!p)
+        return fail(ctx, OSTFT_ERR_INVALID_ARG, "null params");
+    if (p->exp < 6 || p->exp > 20)
+        return fail(ctx, OSTFT_ERR_UNSUPPORTED, "Operation not supported: window exponent must be 6..20");
+    if (!(p->overlap >= 0.0f && p->overlap < 1.0f))
+        return fail(ctx, OSTFT_ERR_INVALID_ARG, "overlap ratio must be in [0, 1)");
+    if (p->window < OSTFT_WIN_NONE || p->window > OSTFT_WIN_GAUSSIAN)
+        return fail(ctx, OSTFT_ERR_INVALID_ARG, "unknown window kind");
+    if (p->output < OSTFT_OUT_COMPAT_POWER || p->output > OSTFT_OUT_HALF_SPECTRUM)
+        return fail(ctx, OSTFT_ERR_INVALID_ARG, "unknown output layout");
+    ostft_geometry& g = plan->g;
+    const uint64_t N = 1ull << p->exp;
+    const double q = to_quot_f32((float)L, p->overlap, (float)N);
+    if (!(q >= 1.0) || q > 2147483647.0)
+        return fail(ctx, OSTFT_ERR_TOO_LARGE, "frame count outside the int32 range of toQuot");
+    g.full_size = L;
+    g.window_size = N;
+    g.frames = (uint64_t)q;
+    g.hop = omove_f32((float)N, p->overlap);
+    g.out_full = g.frames * N;
+    g.out_half = g.out_full / 2;
+    switch (p->output) {
+    case OSTFT_OUT_POWER: g.out_floats = g.frames * (N / 2 + 1); break;
+    case OSTFT_OUT_HALF_SPECTRUM: g.out_floats = g.frames * (N + 2); break;
+    default: g.out_floats = g.out_full; break;
+    }
+    g.compute_frames = p->output == OSTFT_OUT_HALF_COMPLEX ? (g.out_half + N - 1) / N : g.frames;
+    const bool quirk = !(p->flags & OSTFT_FLAG_EXACT_ODD) && p->exp >= 13 && (p->exp & 1)
+                       && (p->output == OSTFT_OUT_COMPAT_POWER || p->output == OSTFT_OUT_HALF_COMPLEX
+                           || p->output == OSTFT_OUT_MAGNITUDE);
+    if (quirk) {
+        plan->pack = ostft::PACK_EVEN_ODD;
+        plan->log_m = p->exp - 1;
+        plan->groups = g.compute_frames;
+    } else {
+        plan->pack = ostft::PACK_PAIR;
+        plan->log_m = p->exp;
+        plan->groups = (g.compute_frames + 1) / 2;
+    }
+    return OSTFT_OK;
+}
+
+// W_m^j for j < count, generated once and kept.
+float2* get_twiddles(ostft_ctx* ctx, int64_t m, int64_t count, int* rc)
+{
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    const auto key = std::make_pair(m, count);
+    auto it = ctx->twiddles.find(key);
+    if (it != ctx->twiddles.end())
+        return it->second;
+    float2* t = nullptr;
+    hipError_t e = hipMalloc(&t, sizeof(float2) * (size_t)count);
+    if (e == hipSuccess)
+        e = ostft::launch_twiddle_table(t, m, count, ctx->stream);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        if (t)
+            (void)hipFree(t);
+        *rc = hip_fail(ctx, e, "twiddle table");
+        return nullptr;
+    }
+    ctx->twiddles[key] = t;
+    return t;
+}
+
+// Window coefficients exactly as the reference computes them on the CPU
+// (OKL/kernel.okl:33-127 as translated in StandAlone/cross_gpgpu/Serial/kernel/
+// compiled.hpp:20-88): the angle is formed in double and stored as float,
+// std::cos/std::exp take that float and return float (the C library's cosf /
+// expf), the weighted sum is double, the result is float.  Evaluated once per
+// (kind, N, sigma) so the table equals the Serial runner's coefficients bit for
+// bit; the GPU then reads it from HBM.
+#pragma clang fp contract(off)
+float compat_window(int kind, int index, int windowSize, float sigma)
+{
+    constexpr double PI = 3.14159265358979323846;
+    switch (kind) {
+    case OSTFT_WIN_HANNING: {
+        const float angle = 2.0 * PI * index / ((float)(windowSize - 1));
+        return 0.5 * (1.0 - std::cos(angle));
+    }
+    case OSTFT_WIN_HAMMING: {
+        const float angle = 2.0 * PI * index / ((float)(windowSize - 1));
+        return 0.54 - (0.46 * std::cos(angle));
+    }
+    case OSTFT_WIN_GAUSSIAN: {
+        const int hw = windowSize >> 1;
+        float angle = ((float)(index - hw)) / ((float)(hw * sigma));
+        angle *= angle;
+        angle *= -0.5;
+        return std::exp(angle);
+    }
+    default:
+        break;
+    }
+    const float f = 2.0 * PI * index / ((float)(windowSize));
+    const float s = f * 2.0;
+    const float t = f * 3.0;
+    const float q = f * 4.0;
+    switch (kind) {
+    case OSTFT_WIN_BLACKMAN: return 0.42 - 0.5 * std::cos(f) + 0.08 * std::cos(s);
+    case OSTFT_WIN_NUTTALL:
+        return 0.355768 - 0.487396 * std::cos(f) + 0.144232 * std::cos(s) - 0.012604 * std::cos(t);
+    case OSTFT_WIN_BLACKMAN_NUTTALL:
+        return 0.3635819 - 0.4891775 * std::cos(f) + 0.1365995 * std::cos(s) - 0.0106411 * std::cos(t);
+    case OSTFT_WIN_BLACKMAN_HARRIS:
+        return 0.35875 - 0.48829 * std::cos(f) + 0.14128 * std::cos(s) - 0.01168 * std::cos(t);
+    case OSTFT_WIN_FLATTOP:
+        return 0.21557895 - 0.41663158 * std::cos(f) + 0.277263158 * std::cos(s) - 0.083578947 * std::cos(t)
+               + 0.006947368 * std::cos(q);
+    default: return 1.0f;
+    }
+}
+
+// Window table for (kind, n, sigma): the reference's coefficients, or with
+// `exact` the FP64 formulas generated on the GPU.
+float* get_window(ostft_ctx* ctx, int kind, int64_t n, float sigma, int exact, int* rc)
+{
+    uint32_t sb = 0;
+    std::memcpy(&sb, &sigma, sizeof(sb));
+    if (kind != OSTFT_WIN_GAUSSIAN)
+        sb = 0;
+    const auto key = std::make_tuple(kind, n, sb, exact);
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    auto it = ctx->windows.find(key);
+    if (it != ctx->windows.end())
+        return it->second;
+    float* w = nullptr;
+    hipError_t e = hipMalloc(&w, sizeof(float) * (size_t)n);
+    if (e == hipSuccess) {
+        if (exact) {
+            e = ostft::launch_window_table(w, n, kind, sigma, ctx->stream);
+        } else {
+            std::vector<float> host((size_t)n);
+            for (int64_t i = 0; i < n; ++i)
+                host[(size_t)i] = compat_window(kind, (int)i, (int)n, sigma);
+            e = hipMemcpyAsync(w, host.data(), sizeof(float) * (size_t)n, hipMemcpyHostToDevice, ctx->stream);
+            if (e == hipSuccess)
+                e = hipStreamSynchronize(ctx->stream);
+        }
+    }
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        if (w)
+            (void)hipFree(w);
+        *rc = hip_fail(ctx, e, "window table");
+        return nullptr;
+    }
+    ctx->windows[key] = w;
+    return w;
+}
+
+}  // namespace
+
+namespace ostft {
+// used by multipass.hip
+float2* ctx_twiddles(ostft_ctx* ctx, int64_t m, int64_t count, int* rc) { return get_twiddles(ctx, m, count, rc); }
+Scratch& ctx_scratch(ostft_ctx* ctx) { return ctx->scratch; }
+}  // namespace ostft
+
+extern "C" {
+
+int ostft_abi_version(void) { return OSTFT_ABI_VERSION; }
+
+const char* ostft_strerror(int status)
+{
+    switch (status) {
+    case OSTFT_OK: return "OK";
+    case OSTFT_ERR_INVALID_ARG: return "invalid argument";
+    case OSTFT_ERR_UNSUPPORTED: return "Operation not supported";
+    case OSTFT_ERR_OPTION: return "--gaussian_window";
+    case OSTFT_ERR_CAPACITY: return "output buffer too small";
+    case OSTFT_ERR_HIP: return "HIP runtime error";
+    case OSTFT_ERR_NO_DEVICE: return "no such HIP device";
+    case OSTFT_ERR_TOO_LARGE: return "request too large";
+    default: return "unknown status";
+    }
+}
+
+const char* ostft_last_error(const ostft_ctx* ctx) { return ctx ? ctx->last_error.c_str() : ""; }
+
+int ostft_window_coefficients(int32_t kind, int32_t n, float sigma, float* out)
+{
+    if (!out || n < 2 || kind < OSTFT_WIN_NONE || kind > OSTFT_WIN_GAUSSIAN)
+        return OSTFT_ERR_INVALID_ARG;
+    for (int32_t i = 0; i < n; ++i)
+        out[i] = compat_window(kind, i, n, sigma);
+    return OSTFT_OK;
+}
+
+// Default_Pipeline's option dispatch, RunnerCommon.cpp:216-248 and :309-314.
+int ostft_parse_options(const char* options, int32_t exp, float overlap, ostft_params* out)
+{
+    if (!out)
+        return OSTFT_ERR_INVALID_ARG;
+    const char* s = options ? options : "";
+    ostft_params p;
+    std::memset(&p, 0, sizeof(p));
+    p.exp = exp;
+    p.overlap = overlap;
+    p.window = OSTFT_WIN_NONE;
+    p.output = OSTFT_OUT_COMPAT_POWER;
+    if (contains(s, "--exact_odd_exp"))
+        p.flags |= OSTFT_FLAG_EXACT_ODD;
+    if (contains(s, "--exact_window"))
+        p.flags |= OSTFT_FLAG_EXACT_WINDOW;
+    if (contains(s, "--exact_sigma_parse"))
+        p.flags |= OSTFT_FLAG_SIGMA_EXACT;
+    static const char* const keys[7] = {"--hanning_window",        "--hamming_window",
+                                        "--blackman_window",       "--nuttall_window",
+                                        "--blackman_nuttall_window", "--blackman_harris_window",
+                                        "--flattop_window"};
+    for (int k = 0; k < 7; ++k) {
+        if (contains(s, keys[k])) {
+            p.window = k + 1;
+            break;
+        }
+    }
+    if (p.window == OSTFT_WIN_NONE) {
+        const char* gk = std::strstr(s, "--gaussian_window=");
+        const char* sk = std::strstr(s, "<<sigma");
+        if (gk && sk) {
+            // std::string::substr(P1, P2 - P1) with P1 = pos + 19, one past the
+            // first sigma character (or pos + 18 with --exact_sigma_parse)
+            const size_t len = std::strlen(s);
+            const size_t p1 = (size_t)(gk - s) + ((p.flags & OSTFT_FLAG_SIGMA_EXACT) ? 18u : 19u);
+            const size_t p2 = (size_t)(sk - s);
+            if (p1 > len)
+                return OSTFT_ERR_OPTION;
+            size_t cnt = p2 - p1;  // wraps like size_t in the reference
+            if (cnt > len - p1)
+                cnt = len - p1;
+            std::string text(s + p1, cnt);
+            char* end = nullptr;
+            errno = 0;
+            const float sigma = std::strtof(text.c_str(), &end);  // std::stof
+            if (end == text.c_str() || errno == ERANGE)
+                return OSTFT_ERR_OPTION;
+            if (sigma > 0.0f) {
+                p.window = OSTFT_WIN_GAUSSIAN;
+                p.sigma = sigma;
+            }
+        }
+    }
+    if (contains(s, "--remove_dc"))
+        p.flags |= OSTFT_FLAG_REMOVE_DC;
+    if (contains(s, "--half_complex_return"))
+        p.output = OSTFT_OUT_HALF_COMPLEX;
+    else if (contains(s, "--half_spectrum_return"))
+        p.output = OSTFT_OUT_HALF_SPECTRUM;
+    else if (contains(s, "--power_return"))
+        p.output = OSTFT_OUT_POWER;
+    else if (contains(s, "--magnitude_return"))
+        p.output = OSTFT_OUT_MAGNITUDE;
+    *out = p;
+    return OSTFT_OK;
+}
+
+int ostft_get_geometry(const ostft_params* params, uint64_t full_size, ostft_geometry* out)
+{
+    if (!out)
+        return OSTFT_ERR_INVALID_ARG;
+    Plan plan;
+    const int rc = make_plan(params, full_size, &plan, nullptr);
+    if (rc == OSTFT_OK)
+        *out = plan.g;
+    return rc;
+}
+
+int ostft_create(int32_t device, ostft_ctx** out)
+{
+    if (!out)
+        return OSTFT_ERR_INVALID_ARG;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n)
+        return OSTFT_ERR_NO_DEVICE;
+    DeviceGuard dg(device);
+    if (!dg.ok)
+        return OSTFT_ERR_NO_DEVICE;
+    ostft_ctx* ctx = new ostft_ctx();
+    ctx->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete ctx;
+        return OSTFT_ERR_HIP;
+    }
+    *out = ctx;
+    return OSTFT_OK;
+}
+
+void ostft_destroy(ostft_ctx* ctx)
+{
+    if (!ctx)
+        return;
+    {
+        DeviceGuard dg(ctx->device);
+        (void)hipDeviceSynchronize();
+        for (auto& kv : ctx->twiddles)
+            (void)hipFree(kv.second);
+        for (auto& kv : ctx->windows)
+            (void)hipFree(kv.second);
+        if (ctx->d_in)
+            (void)hipFree(ctx->d_in);
+        if (ctx->d_out)
+            (void)hipFree(ctx->d_out);
+        ctx->scratch.release();
+        if (ctx->stream)
+            (void)hipStreamDestroy(ctx->stream);
+    }
+    delete ctx;
+}
+
+int ostft_run_device(ostft_ctx* ctx, const ostft_params* params, const float* d_in, uint64_t full_size,
+                     float* d_out, uint64_t out_cap, void* hip_stream)
+{
+    if (!ctx || !params || !d_out || (!d_in && full_size > 0))
+        return fail(ctx, OSTFT_ERR_INVALID_ARG, "null argument");
+    Plan plan;
+    int rc = make_plan(params, full_size, &plan, ctx);
+    if (rc != OSTFT_OK)
+        return rc;
+    if (out_cap < plan.g.out_floats)
+        return fail(ctx, OSTFT_ERR_CAPACITY, "output buffer holds " + std::to_string(out_cap) + " floats, need "
+                                                 + std::to_string(plan.g.out_floats));
+    DeviceGuard dg(ctx->device);
+    if (!dg.ok)
+        return fail(ctx, OSTFT_ERR_NO_DEVICE, "hipSetDevice failed");
+    hipStream_t s = static_cast<hipStream_t>(hip_stream);
+    const uint64_t N = plan.g.window_size;
+
+    int win_kind = params->window;
+    if (win_kind == OSTFT_WIN_GAUSSIAN && !(params->sigma > 0.0f))
+        win_kind = OSTFT_WIN_NONE;  // the reference skips the window for sigma <= 0
+    const float* w = nullptr;
+    if (win_kind != OSTFT_WIN_NONE) {
+        w = get_window(ctx, win_kind, (int64_t)N, params->sigma, (params->flags & OSTFT_FLAG_EXACT_WINDOW) ? 1 : 0,
+                       &rc);
+        if (!w)
+            return rc;
+    }
+    const int64_t M = 1ll << plan.log_m;
+
+    ostft::FusedArgs a;
+    a.in = d_in;
+    a.L = full_size;
+    a.hop = plan.g.hop;
+    a.frames = plan.g.compute_frames;
+    a.groups = plan.groups;
+    a.win = w;
+    a.out = d_out;
+    a.out_limit = plan.g.out_half;
+    a.pack = plan.pack;
+    a.out_mode = params->output;
+    a.dc = (params->flags & OSTFT_FLAG_REMOVE_DC) ? 1 : 0;
+
+    if (plan.groups == 0)
+        return OSTFT_OK;
+    if (ostft::fused_supported(plan.log_m)) {
+        a.tw = get_twiddles(ctx, M, M, &rc);
+        if (!a.tw)
+            return rc;
+        hipError_t e = ostft::launch_fused(plan.log_m, a, s);
+        if (e != hipSuccess)
+            return hip_fail(ctx, e, "fused STFT launch");
+        return OSTFT_OK;
+    }
+    return ostft::run_multipass(ctx, plan.log_m, a, s);
+}
+
+int ostft_activate(ostft_ctx* ctx, const float* in, uint64_t full_size, int32_t exp, float overlap,
+                   const char* options, float* out, uint64_t out_cap, uint64_t* out_len)
+{
+    if (!ctx || !out || (!in && full_size > 0))
+        return fail(ctx, OSTFT_ERR_INVALID_ARG, "null argument");
+    ostft_params p;
+    int rc = ostft_parse_options(options, exp, overlap, &p);
+    if (rc != OSTFT_OK)
+        return fail(ctx, rc, "--gaussian_window");
+    Plan plan;
+    rc = make_plan(&p, full_size, &plan, ctx);
+    if (rc != OSTFT_OK)
+        return rc;
+    if (out_cap < plan.g.out_floats)
+        return fail(ctx, OSTFT_ERR_CAPACITY, "output buffer too small");
+    DeviceGuard dg(ctx->device);
+    if (!dg.ok)
+        return fail(ctx, OSTFT_ERR_NO_DEVICE, "hipSetDevice failed");
+    hipError_t e = hipSuccess;
+    const uint64_t in_n = full_size > 0 ? full_size : 1;
+    if (ctx->d_in_cap < in_n) {
+        if (ctx->d_in)
+            (void)hipFree(ctx->d_in);
+        ctx->d_in = nullptr;
+        ctx->d_in_cap = 0;
+        e = hipMalloc(&ctx->d_in, sizeof(float) * in_n);
+        if (e != hipSuccess)
+            return hip_fail(ctx, e, "input staging");
+        ctx->d_in_cap = in_n;
+    }
+    if (ctx->d_out_cap < plan.g.out_floats) {
+        if (ctx->d_out)
+            (void)hipFree(ctx->d_out);
+        ctx->d_out = nullptr;
+        ctx->d_out_cap = 0;
+        e = hipMalloc(&ctx->d_out, sizeof(float) * plan.g.out_floats);
+        if (e != hipSuccess)
+            return hip_fail(ctx, e, "output staging");
+        ctx->d_out_cap = plan.g.out_floats;
+    }
+    if (full_size > 0) {
+        e = hipMemcpyAsync(ctx->d_in, in, sizeof(float) * full_size, hipMemcpyHostToDevice, ctx->stream);
+        if (e != hipSuccess)
+            return hip_fail(ctx, e, "H2D copy");
+    }
+    rc = ostft_run_device(ctx, &p, ctx->d_in, full_size, ctx->d_out, ctx->d_out_cap, ctx->stream);
+    if (rc != OSTFT_OK)
+        return rc;
+    e = hipMemcpyAsync(out, ctx->d_out, sizeof(float) * plan.g.out_floats, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess)
+        return hip_fail(ctx, e, "D2H copy");
+    if (out_len)
+        *out_len = plan.g.out_floats;
+    return OSTFT_OK;
+}
+
+}  // extern "C"

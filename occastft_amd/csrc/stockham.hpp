@@ -1,0 +1,121 @@
+// stockham.hpp -- the LDS-exchanged Stockham engine shared by the fused
+// single-kernel path and the multi-pass column kernels.
+//
+// A "group" of T = M/16 threads owns one complex transform of size M = 2^LOG_M
+// with 16 values per thread in VGPRs.  Passes are radix 16 (plus one final
+// radix 2/4/8 pass); pass PASS with radix r and stride Ns (product of the
+// earlier radices) does, for every butterfly g < M/r,
+//     v[m] = X[g + m*M/r] * W_{Ns*r}^{(g mod Ns)*m},   v = DFT_r(v),
+//     Y[(g/Ns)*Ns*r + (g mod Ns) + q*Ns] = v[q],
+// which leaves the spectrum in natural order (Stockham autosort).
+#pragma once
+
+#include "fft_device.hpp"
+
+namespace ostft {
+
+template <int LOG_M>
+struct Cfg {
+    static constexpr int M = 1 << LOG_M;
+    static constexpr int T = M / 16;               // threads per transform
+    static constexpr int WG = T < 256 ? 256 : T;   // threads per workgroup
+    static constexpr int G = WG / T;               // transforms per workgroup
+    static constexpr int PADM = M + M / 16;        // padded LDS length per transform
+    static constexpr int NP16 = LOG_M / 4;
+    static constexpr int REM = LOG_M % 4;
+    static constexpr int NPASS = NP16 + (REM ? 1 : 0);
+    static constexpr int radix(int i) { return i < NP16 ? 16 : (1 << REM); }
+    static constexpr int ns(int i)
+    {
+        int n = 1;
+        for (int k = 0; k < i; ++k)
+            n *= radix(k);
+        return n;
+    }
+};
+
+__device__ __forceinline__ uint32_t pad(uint32_t i) { return i + (i >> 4); }
+
+// Workgroup-order remap: hardware hands consecutive workgroups to the 8 XCDs
+// round-robin; this gives XCD x the contiguous logical range it would own.
+__device__ __forceinline__ uint32_t xcd_block(uint32_t bid, uint32_t nb)
+{
+    const uint32_t xcd = bid & 7u, idx = bid >> 3, q = nb >> 3, r = nb & 7u;
+    return xcd < r ? xcd * (q + 1u) + idx : r * (q + 1u) + (xcd - r) * q + idx;
+}
+
+// Sum over the T consecutive lanes of a group (all threads must call it).
+template <int T, int WG>
+__device__ __forceinline__ float2 group_sum(float2 s, float2* red, uint32_t tid)
+{
+    constexpr int W = T < 64 ? T : 64;
+#pragma unroll
+    for (int off = W / 2; off > 0; off >>= 1) {
+        s.x += __shfl_xor(s.x, off, 64);
+        s.y += __shfl_xor(s.y, off, 64);
+    }
+    if constexpr (T > 64) {
+        constexpr int WPG = T / 64;
+        const uint32_t wave = tid >> 6;
+        if ((tid & 63u) == 0)
+            red[wave] = s;
+        __syncthreads();
+        const uint32_t w0 = (wave / WPG) * WPG;
+        float2 acc = red[w0];
+#pragma unroll
+        for (int i = 1; i < WPG; ++i)
+            acc = cadd(acc, red[w0 + i]);
+        s = acc;
+    }
+    return s;
+}
+
+// Stockham pass PASS: twiddle, radix-r DFTs, scatter to LDS; then gather the
+// inputs of the next pass (or stop with the spectrum in LDS).
+template <int LOG_M, int PASS>
+__device__ __forceinline__ void stockham(float2 (&v)[16], float2* z, const float2* __restrict__ tw, uint32_t t)
+{
+    using C = Cfg<LOG_M>;
+    constexpr int M = C::M, T = C::T;
+    constexpr int r = C::radix(PASS), NS = C::ns(PASS), U = 16 / r;
+    if constexpr (NS > 1) {
+        constexpr int STEP = M / (NS * r);
+        cfor<0, U>([&](auto u) {
+            constexpr int UU = decltype(u)::value;
+            const uint32_t kk = (t + UU * T) & (NS - 1);
+            cfor<1, r>([&](auto m) {
+                constexpr int MM = decltype(m)::value;
+                v[UU * r + MM] = cmul(v[UU * r + MM], tw[kk * (MM * STEP)]);
+            });
+        });
+    }
+    cfor<0, U>([&](auto u) {
+        constexpr int UU = decltype(u)::value;
+        dft<r, UU * r>(v);
+    });
+    cfor<0, U>([&](auto u) {
+        constexpr int UU = decltype(u)::value;
+        const uint32_t g = t + UU * T;
+        const uint32_t base = (g / NS) * (NS * r) + (g & (NS - 1));
+        cfor<0, r>([&](auto q) {
+            constexpr int QQ = decltype(q)::value;
+            z[pad(base + QQ * NS)] = v[UU * r + QQ];
+        });
+    });
+    __syncthreads();
+    if constexpr (PASS + 1 < C::NPASS) {
+        constexpr int r2 = C::radix(PASS + 1), U2 = 16 / r2;
+        cfor<0, U2>([&](auto u) {
+            constexpr int UU = decltype(u)::value;
+            const uint32_t g = t + UU * T;
+            cfor<0, r2>([&](auto m) {
+                constexpr int MM = decltype(m)::value;
+                v[UU * r2 + MM] = z[pad(g + MM * (M / r2))];
+            });
+        });
+        __syncthreads();
+        stockham<LOG_M, PASS + 1>(v, z, tw, t);
+    }
+}
+
+}  // namespace ostft
