@@ -90,9 +90,8 @@ uint64_t omove_f32(float windowSize, float overlapRatio)
 
 struct Plan {
     ostft_geometry g;
-    int pack;       // ostft::PackMode
-    int log_m;      // complex transform size
-    uint64_t groups;
+    int log_m;    // complex transform size M = N/2
+    int combine;  // 0: stop before the last radix-2 stage (odd-exponent result)
 };
 
 int make_plan(const ostft_params* p, uint64_t L, Plan* plan, ostft_ctx* ctx)
@@ -127,15 +126,8 @@ int make_plan(const ostft_params* p, uint64_t L, Plan* plan, ostft_ctx* ctx)
     const bool quirk = !(p->flags & OSTFT_FLAG_EXACT_ODD) && p->exp >= 13 && (p->exp & 1)
                        && (p->output == OSTFT_OUT_COMPAT_POWER || p->output == OSTFT_OUT_HALF_COMPLEX
                            || p->output == OSTFT_OUT_MAGNITUDE);
-    if (quirk) {
-        plan->pack = ostft::PACK_EVEN_ODD;
-        plan->log_m = p->exp - 1;
-        plan->groups = g.compute_frames;
-    } else {
-        plan->pack = ostft::PACK_PAIR;
-        plan->log_m = p->exp;
-        plan->groups = (g.compute_frames + 1) / 2;
-    }
+    plan->log_m = p->exp - 1;
+    plan->combine = quirk ? 0 : 1;
     return OSTFT_OK;
 }
 
@@ -447,15 +439,18 @@ int ostft_run_device(ostft_ctx* ctx, const ostft_params* params, const float* d_
     a.L = full_size;
     a.hop = plan.g.hop;
     a.frames = plan.g.compute_frames;
-    a.groups = plan.groups;
     a.win = w;
     a.out = d_out;
     a.out_limit = plan.g.out_half;
-    a.pack = plan.pack;
+    a.combine = plan.combine;
     a.out_mode = params->output;
     a.dc = (params->flags & OSTFT_FLAG_REMOVE_DC) ? 1 : 0;
+    a.tw = nullptr;
+    a.tw_n = get_twiddles(ctx, 2 * M, M, &rc);  // W_N^k, k < M
+    if (!a.tw_n)
+        return rc;
 
-    if (plan.groups == 0)
+    if (a.frames == 0)
         return OSTFT_OK;
     if (ostft::fused_supported(plan.log_m)) {
         a.tw = get_twiddles(ctx, M, M, &rc);

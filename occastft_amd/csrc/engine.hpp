@@ -8,17 +8,6 @@
 
 namespace ostft {
 
-// How a complex FFT of size M is fed from real frames.
-enum PackMode : int32_t {
-    // z = frame(2P) + i * frame(2P+1): two real frames of N = M samples share
-    // one complex transform; the spectra are split in the epilogue.
-    PACK_PAIR = 0,
-    // z[n] = x[2n] + i * x[2n+1] of one frame of N = 2M samples: the split gives
-    // [DFT_M(even samples), DFT_M(odd samples)], which is exactly the radix-2
-    // Stockham buffer one stage before the end (the reference's odd-window result).
-    PACK_EVEN_ODD = 1,
-};
-
 // output layouts (values of enum ostft_output in occastft_hip.h)
 enum OutMode : int32_t {
     OUT_COMPAT_POWER = 0,
@@ -28,26 +17,26 @@ enum OutMode : int32_t {
     OUT_HALF_SPECTRUM = 4,
 };
 
+// One request, as the kernels see it.  Each frame of N = 2M real samples is
+// one complex transform of M points (even samples real, odd samples imaginary).
 struct FusedArgs {
     const float* in;      // signal, L floats
     uint64_t L;
     uint64_t hop;         // OMove
     uint64_t frames;      // frames to transform
-    uint64_t groups;      // complex transforms: ceil(frames/2) (pair) or frames (even/odd)
     const float* win;     // N window coefficients, or nullptr for none
     const float2* tw;     // W_M^j, j in [0, M)
+    const float2* tw_n;   // W_N^k, k in [0, M): the final radix-2 stage
     float* out;
     uint64_t out_limit;   // HALF_COMPLEX: complex values to write (OHalfSize)
-    int32_t pack;         // PackMode
+    int32_t combine;      // 1: full FFT; 0: stop one radix-2 stage early (odd-exponent result)
     int32_t out_mode;     // OutMode
     int32_t dc;           // subtract the frame mean after windowing
 };
 
-// Single-kernel path: complex size M = 2^log_m, 6 <= log_m <= 13.
+// Single-kernel path: complex size M = 2^log_m, 5 <= log_m <= 13 (N = 64 .. 16384).
 bool fused_supported(int log_m);
 hipError_t launch_fused(int log_m, const FusedArgs& a, hipStream_t s);
-// The largest group count one launch handles (grid-size bound).
-uint64_t fused_max_groups(int log_m);
 
 // FP64 table generators (run once per table).
 // tw[j] = exp(-2 pi i j / m) for j < count (count <= m)

@@ -1,0 +1,94 @@
+// epilogue.hpp -- from the packed half-length spectrum to the output layouts.
+//
+// A real frame x[0..N) is transformed as one complex sequence of M = N/2
+// points, z[n] = x[2n] + i x[2n+1].  With Z = DFT_M(z):
+//     E[k] = (Z[k] + conj Z[M-k]) / 2          = DFT_M(x[0::2])[k]
+//     O[k] = (Z[k] - conj Z[M-k]) / (2i)       = DFT_M(x[1::2])[k]
+// The last radix-2 stage of the N-point transform is
+//     X[k] = E[k] + W_N^k O[k],   X[k+M] = E[k] - W_N^k O[k],
+// and the buffer one stage earlier -- what the reference returns for odd
+// window exponents >= 13 -- is simply [E, O].  Either way bin k of the packed
+// spectrum yields the outputs at positions k and M+k of the frame.
+#pragma once
+
+#include "engine.hpp"
+#include "fft_device.hpp"
+
+namespace ostft {
+
+struct BinPair {
+    float2 lo;  // position k
+    float2 hi;  // position M + k
+};
+
+// combine != 0: final radix-2 stage with w = W_N^k; otherwise [E, O].
+// dc0 != 0 (only for k == 0): remove the frame mean in the frequency domain,
+// which moves (E0, O0) to ((E0-O0)/2, (O0-E0)/2).
+__device__ __forceinline__ BinPair finish_bin(float2 zk, float2 zm, float2 w, int combine, bool dc0)
+{
+    float2 E = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
+    float2 O = make_float2(0.5f * (zk.y + zm.y), 0.5f * (zm.x - zk.x));
+    if (dc0) {
+        const float d = 0.5f * (E.x - O.x);
+        E = make_float2(d, 0.0f);
+        O = make_float2(-d, 0.0f);
+    }
+    BinPair r;
+    if (combine) {
+        const float2 t = cmul(w, O);
+        r.lo = cadd(E, t);
+        r.hi = csub(E, t);
+    } else {
+        r.lo = E;
+        r.hi = O;
+    }
+    return r;
+}
+
+// Write the outputs of bin k of frame f (frame length N = 2M).
+//   COMPAT_POWER  sqrt(Re^2 + Re^2) at f*N + k and f*N + M + k
+//   MAGNITUDE     |.| at the same places
+//   HALF_COMPLEX  (Re, Im) at complex index f*N + k, f*N + M + k, below `limit`
+//   POWER         |X|^2 at f*(M+1) + k, and X[M] from k == 0
+//   HALF_SPECTRUM (Re, Im) at complex index f*(M+1) + k, and X[M] from k == 0
+__device__ __forceinline__ void write_bin(float* __restrict__ out, int mode, uint64_t f, uint32_t M, uint32_t k,
+                                          const BinPair& b, uint64_t limit)
+{
+    const uint64_t base = f * (uint64_t)(2 * M);
+    switch (mode) {
+    case OUT_COMPAT_POWER:
+        nt_store(sqrtf(b.lo.x * b.lo.x + b.lo.x * b.lo.x), out + base + k);
+        nt_store(sqrtf(b.hi.x * b.hi.x + b.hi.x * b.hi.x), out + base + M + k);
+        break;
+    case OUT_MAGNITUDE:
+        nt_store(sqrtf(b.lo.x * b.lo.x + b.lo.y * b.lo.y), out + base + k);
+        nt_store(sqrtf(b.hi.x * b.hi.x + b.hi.y * b.hi.y), out + base + M + k);
+        break;
+    case OUT_HALF_COMPLEX: {
+        float2* o2 = reinterpret_cast<float2*>(out);
+        if (base + k < limit)
+            nt_store(b.lo, o2 + base + k);
+        if (base + M + k < limit)
+            nt_store(b.hi, o2 + base + M + k);
+        break;
+    }
+    case OUT_POWER: {
+        float* o = out + f * (uint64_t)(M + 1);
+        nt_store(b.lo.x * b.lo.x + b.lo.y * b.lo.y, o + k);
+        if (k == 0)
+            nt_store(b.hi.x * b.hi.x + b.hi.y * b.hi.y, o + M);
+        break;
+    }
+    case OUT_HALF_SPECTRUM: {
+        float2* o = reinterpret_cast<float2*>(out) + f * (uint64_t)(M + 1);
+        nt_store(b.lo, o + k);
+        if (k == 0)
+            nt_store(b.hi, o + M);
+        break;
+    }
+    default:
+        break;
+    }
+}
+
+}  // namespace ostft

@@ -11,11 +11,15 @@
 // Stockham engine as the fused kernel, and scatters the Stockham outputs
 // Y[(g/Ns)Ns R + (g mod Ns) + q Ns] in CB-wide contiguous rows.
 //
-// The first pass reads the signal itself (overlap gather + window); the last
-// pass leaves the packed spectrum Z in natural order, and a split kernel reads
-// Z[k], Z[M-k] to produce the requested output.  DC removal is applied in the
-// frequency domain there: subtracting the frame mean only changes bin 0.
+// As in the fused kernel, a frame of N real samples is one complex transform
+// of M = N/2 points (even samples real, odd imaginary).  The first pass reads
+// the signal itself (overlap gather + window); the last pass leaves the packed
+// spectrum Z in natural order, and a split kernel reads Z[k], Z[M-k], applies
+// the final radix-2 stage (epilogue.hpp) and writes the requested output.  DC
+// removal is applied in the frequency domain there: subtracting the frame mean
+// only changes the k = 0 terms.
 #include "../../include/occastft_hip.h"
+#include "epilogue.hpp"
 #include "multipass.hpp"
 #include "stockham.hpp"
 
@@ -53,7 +57,6 @@ struct PassArgs {
     const float* sig;  // first pass: the signal
     uint64_t L, hop, frames;
     const float* win;
-    int32_t pack;
     int32_t first;
     const float2* src;       // later passes: previous pass output
     float2* dst;
@@ -63,7 +66,6 @@ struct PassArgs {
     uint32_t fine_bits;
     uint32_t log_m;
     uint32_t log_ns;         // log2 of the pass stride Ns
-    uint64_t groups;
 };
 
 template <int LOG_R>
@@ -78,7 +80,7 @@ __global__ void __launch_bounds__(256) column_pass_kernel(const PassArgs a)
     const uint64_t bpg = M >> 12;  // blocks per transform = (M / R) / CB
     const uint64_t blk = xcd_block(blockIdx.x, gridDim.x);
     const uint64_t P = blk / bpg;
-    if (P >= a.groups)
+    if (P >= a.frames)
         return;  // uniform over the block
     const uint32_t g0 = (uint32_t)(blk - P * bpg) * CB;
     const uint32_t stride = (uint32_t)(M >> LOG_R);
@@ -91,20 +93,11 @@ __global__ void __launch_bounds__(256) column_pass_kernel(const PassArgs a)
         const uint32_t n = g0 + c + m * stride;
         float2 x;
         if (a.first) {
-            if (a.pack == PACK_PAIR) {
-                const uint64_t fa = 2 * P, fb = fa + 1;
-                const float w = a.win ? a.win[n] : 1.0f;
-                const uint64_t sa = fa * a.hop + n, sb = fb * a.hop + n;
-                const float xa = sa < a.L ? a.sig[sa] : 0.0f;
-                const float xb = (fb < a.frames && sb < a.L) ? a.sig[sb] : 0.0f;
-                x = make_float2(xa * w, xb * w);
-            } else {
-                const uint64_t s0 = P * a.hop + 2ull * n;
-                const float2 w = a.win ? reinterpret_cast<const float2*>(a.win)[n] : make_float2(1.0f, 1.0f);
-                const float x0 = s0 < a.L ? a.sig[s0] : 0.0f;
-                const float x1 = s0 + 1 < a.L ? a.sig[s0 + 1] : 0.0f;
-                x = make_float2(x0 * w.x, x1 * w.y);
-            }
+            const uint64_t s0 = P * a.hop + 2ull * n;
+            const float2 w = a.win ? reinterpret_cast<const float2*>(a.win)[n] : make_float2(1.0f, 1.0f);
+            const float x0 = s0 < a.L ? a.sig[s0] : 0.0f;
+            const float x1 = s0 + 1 < a.L ? a.sig[s0 + 1] : 0.0f;
+            x = make_float2(x0 * w.x, x1 * w.y);
         } else {
             x = a.src[P * M + n];
             if (ns > 1) {
@@ -148,73 +141,24 @@ __global__ void __launch_bounds__(256) column_pass_kernel(const PassArgs a)
 
 struct SplitArgs {
     const float2* Z;
+    const float2* tw_n;  // W_N^k, k < M
     float* out;
-    uint64_t groups, frames, out_limit;
+    uint64_t frames, out_limit;
     uint32_t log_m;
-    int32_t pack, out_mode, dc;
+    int32_t combine, out_mode, dc;
 };
 
 __global__ void __launch_bounds__(256) split_kernel(const SplitArgs a)
 {
     const uint64_t M = 1ull << a.log_m;
-    const uint64_t total = a.groups * M;
-    float* __restrict__ out = a.out;
+    const uint64_t total = a.frames * M;
     for (uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x; i < total; i += (uint64_t)gridDim.x * 256u) {
-        const uint64_t P = i >> a.log_m;
+        const uint64_t f = i >> a.log_m;
         const uint32_t k = (uint32_t)(i & (M - 1));
         const uint32_t km = (uint32_t)((M - k) & (M - 1));
-        const float2 zk = a.Z[P * M + k], zm = a.Z[P * M + km];
-        float2 A = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
-        float2 B = make_float2(0.5f * (zk.y + zm.y), 0.5f * (zm.x - zk.x));
-        if (a.dc && k == 0) {
-            if (a.pack == PACK_PAIR) {
-                A = B = make_float2(0.0f, 0.0f);
-            } else {
-                const float e0 = 0.5f * (zk.x - zk.y);
-                A = make_float2(e0, 0.0f);
-                B = make_float2(-e0, 0.0f);
-            }
-        }
-        const bool second = a.pack == PACK_PAIR ? (2 * P + 1 < a.frames) : true;
-        const uint64_t base = P * 2 * M;
-        switch (a.out_mode) {
-        case OUT_COMPAT_POWER:
-            nt_store(sqrtf(A.x * A.x + A.x * A.x), out + base + k);
-            if (second)
-                nt_store(sqrtf(B.x * B.x + B.x * B.x), out + base + M + k);
-            break;
-        case OUT_MAGNITUDE:
-            nt_store(sqrtf(A.x * A.x + A.y * A.y), out + base + k);
-            if (second)
-                nt_store(sqrtf(B.x * B.x + B.y * B.y), out + base + M + k);
-            break;
-        case OUT_HALF_COMPLEX: {
-            float2* o2 = reinterpret_cast<float2*>(out);
-            if (base + k < a.out_limit)
-                nt_store(A, o2 + base + k);
-            if (second && base + M + k < a.out_limit)
-                nt_store(B, o2 + base + M + k);
-            break;
-        }
-        case OUT_POWER:
-            if (k <= M / 2) {
-                float* oa = out + 2 * P * (M / 2 + 1);
-                nt_store(A.x * A.x + A.y * A.y, oa + k);
-                if (second)
-                    nt_store(B.x * B.x + B.y * B.y, oa + (M / 2 + 1) + k);
-            }
-            break;
-        case OUT_HALF_SPECTRUM:
-            if (k <= M / 2) {
-                float2* oa = reinterpret_cast<float2*>(out) + 2 * P * (M / 2 + 1);
-                nt_store(A, oa + k);
-                if (second)
-                    nt_store(B, oa + (M / 2 + 1) + k);
-            }
-            break;
-        default:
-            break;
-        }
+        const float2 w = a.combine ? a.tw_n[k] : make_float2(1.0f, 0.0f);
+        const BinPair b = finish_bin(a.Z[f * M + k], a.Z[f * M + km], w, a.combine, a.dc && k == 0);
+        write_bin(a.out, a.out_mode, f, (uint32_t)M, k, b, a.out_limit);
     }
 }
 
@@ -233,7 +177,7 @@ int multipass_plan(int log_m, int* lr)
 template <int LOG_R>
 static hipError_t launch_pass(const PassArgs& p, hipStream_t s)
 {
-    const uint64_t blocks = p.groups * ((1ull << p.log_m) >> 12);
+    const uint64_t blocks = p.frames * ((1ull << p.log_m) >> 12);
     if (blocks > 0x7fffffffull)
         return hipErrorInvalidValue;
     hipLaunchKernelGGL(column_pass_kernel<LOG_R>, dim3((uint32_t)blocks), dim3(256), 0, s, p);
@@ -242,13 +186,13 @@ static hipError_t launch_pass(const PassArgs& p, hipStream_t s)
 
 int run_multipass(ostft_ctx* ctx, int log_m, const FusedArgs& a, hipStream_t s)
 {
-    if (log_m < 13 || log_m > 20)
+    if (log_m < 13 || log_m > 19)
         return OSTFT_ERR_UNSUPPORTED;
     int lr[4];
     const int np = multipass_plan(log_m, lr);
     const uint64_t M = 1ull << log_m;
     Scratch& sc = ctx_scratch(ctx);
-    hipError_t e = sc.ensure(a.groups * M);
+    hipError_t e = sc.ensure(a.frames * M);
     if (e != hipSuccess)
         return OSTFT_ERR_HIP;
     int rc = OSTFT_OK;
@@ -266,12 +210,10 @@ int run_multipass(ostft_ctx* ctx, int log_m, const FusedArgs& a, hipStream_t s)
     p.hop = a.hop;
     p.frames = a.frames;
     p.win = a.win;
-    p.pack = a.pack;
     p.tw_fine = tw_fine;
     p.tw_coarse = tw_coarse;
     p.fine_bits = fb;
     p.log_m = (uint32_t)log_m;
-    p.groups = a.groups;
     const float2* src = nullptr;
     float2* dst = sc.a;
     uint32_t log_ns = 0;
@@ -298,15 +240,15 @@ int run_multipass(ostft_ctx* ctx, int log_m, const FusedArgs& a, hipStream_t s)
     }
     SplitArgs sa;
     sa.Z = src;
+    sa.tw_n = a.tw_n;
     sa.out = a.out;
-    sa.groups = a.groups;
     sa.frames = a.frames;
     sa.out_limit = a.out_limit;
     sa.log_m = (uint32_t)log_m;
-    sa.pack = a.pack;
+    sa.combine = a.combine;
     sa.out_mode = a.out_mode;
     sa.dc = a.dc;
-    const uint64_t total = a.groups * M;
+    const uint64_t total = a.frames * M;
     uint64_t blocks = (total + 255) / 256;
     if (blocks > 65536ull * 16)
         blocks = 65536ull * 16;
