@@ -127,6 +127,28 @@ def cpu_baseline(x_dev, gpu_runner, budget_s: float):
     }
 
 
+def copy_probe(args) -> int:
+    """torch's device copy of 1 GiB: 2 GiB of HBM traffic per launch."""
+    import torch
+
+    n = 1 << 28
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(args.warmup):
+        dst.copy_(src)
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    s.record()
+    for _ in range(args.steps):
+        dst.copy_(src)
+    e.record()
+    torch.cuda.synchronize()
+    ms = s.elapsed_time(e) / args.steps
+    print(json.dumps({"copy_bytes_per_launch": 8 * n, "ms": round(ms, 4),
+                      "gbps": round(8 * n / (ms / 1e3) / 1e9, 1)}))
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -134,6 +156,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline time budget")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only time a plain 1 GiB device copy (achievable HBM bandwidth; counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: frames are independent, so N GPUs would each run this "
@@ -145,6 +169,8 @@ def main() -> int:
     import occastft_amd as ostft
 
     torch.cuda.set_device(0)
+    if args.copy_probe:
+        return copy_probe(args)
     runner = ostft.Runner(0)
     x = make_signal(SIGNAL_LEN, SEED)
     reqs = []

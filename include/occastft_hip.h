@@ -121,8 +121,8 @@ int ostft_get_geometry(const ostft_params* params, uint64_t full_size, ostft_geo
 int ostft_create(int32_t device, ostft_ctx** out);
 void ostft_destroy(ostft_ctx* ctx);
 
-/* Device-resident STFT: `d_in` (L floats) and `d_out` (out_cap floats) are
- * device pointers on the context's device; work is queued on `hip_stream`
+/* Device-resident STFT: `d_in` (L floats) and `d_out` (out_cap floats, 16-byte
+ * aligned) are device pointers on the context's device; work is queued on `hip_stream`
  * (a hipStream_t, or NULL for the null stream) and this call does not wait
  * for it.  The first call for a new (window, exp) builds the window and
  * FP64-generated twiddle tables synchronously.  Replaces the device part of Runner::ActivateSTFT. */

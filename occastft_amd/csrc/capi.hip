@@ -416,6 +416,8 @@ int ostft_run_device(ostft_ctx* ctx, const ostft_params* params, const float* d_
     if (out_cap < plan.g.out_floats)
         return fail(ctx, OSTFT_ERR_CAPACITY, "output buffer holds " + std::to_string(out_cap) + " floats, need "
                                                  + std::to_string(plan.g.out_floats));
+    if ((reinterpret_cast<uintptr_t>(d_out) & 15u) != 0)
+        return fail(ctx, OSTFT_ERR_INVALID_ARG, "device output must be 16-byte aligned");
     DeviceGuard dg(ctx->device);
     if (!dg.ok)
         return fail(ctx, OSTFT_ERR_NO_DEVICE, "hipSetDevice failed");

@@ -45,6 +45,11 @@ __device__ __forceinline__ BinPair finish_bin(float2 zk, float2 zm, float2 w, in
     return r;
 }
 
+// sqrt(Re^2 + Re^2) of the Serial runner's ToPower, as sqrt(2)|Re|
+__device__ __forceinline__ float compat_power(float2 x) { return 1.41421356237309504880f * fabsf(x.x); }
+// |x| with the hardware square root (1 ulp)
+__device__ __forceinline__ float magnitude(float2 x) { return __builtin_amdgcn_sqrtf(x.x * x.x + x.y * x.y); }
+
 // Write the outputs of bin k of frame f (frame length N = 2M).
 //   COMPAT_POWER  sqrt(Re^2 + Re^2) at f*N + k and f*N + M + k
 //   MAGNITUDE     |.| at the same places
@@ -57,12 +62,12 @@ __device__ __forceinline__ void write_bin(float* __restrict__ out, int mode, uin
     const uint64_t base = f * (uint64_t)(2 * M);
     switch (mode) {
     case OUT_COMPAT_POWER:
-        nt_store(sqrtf(b.lo.x * b.lo.x + b.lo.x * b.lo.x), out + base + k);
-        nt_store(sqrtf(b.hi.x * b.hi.x + b.hi.x * b.hi.x), out + base + M + k);
+        nt_store(compat_power(b.lo), out + base + k);
+        nt_store(compat_power(b.hi), out + base + M + k);
         break;
     case OUT_MAGNITUDE:
-        nt_store(sqrtf(b.lo.x * b.lo.x + b.lo.y * b.lo.y), out + base + k);
-        nt_store(sqrtf(b.hi.x * b.hi.x + b.hi.y * b.hi.y), out + base + M + k);
+        nt_store(magnitude(b.lo), out + base + k);
+        nt_store(magnitude(b.hi), out + base + M + k);
         break;
     case OUT_HALF_COMPLEX: {
         float2* o2 = reinterpret_cast<float2*>(out);

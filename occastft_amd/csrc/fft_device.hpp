@@ -21,11 +21,36 @@ __device__ __forceinline__ float2 cmul(float2 a, float2 b)
 typedef float f32x2 __attribute__((ext_vector_type(2)));
 
 // Streaming stores: outputs are written once and never re-read by the kernel.
-__device__ __forceinline__ void nt_store(float v, float* p) { __builtin_nontemporal_store(v, p); }
+#ifndef OSTFT_NT_STORE
+#define OSTFT_NT_STORE 1
+#endif
+__device__ __forceinline__ void nt_store(float v, float* p)
+{
+#if OSTFT_NT_STORE
+    __builtin_nontemporal_store(v, p);
+#else
+    *p = v;
+#endif
+}
 __device__ __forceinline__ void nt_store(float2 v, float2* p)
 {
     f32x2 t = {v.x, v.y};
+#if OSTFT_NT_STORE
     __builtin_nontemporal_store(t, reinterpret_cast<f32x2*>(p));
+#else
+    *reinterpret_cast<f32x2*>(p) = t;
+#endif
+}
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ void nt_store4(float4 v, float4* p)
+{
+    f32x4 t = {v.x, v.y, v.z, v.w};
+#if OSTFT_NT_STORE
+    __builtin_nontemporal_store(t, reinterpret_cast<f32x4*>(p));
+#else
+    *reinterpret_cast<f32x4*>(p) = t;
+#endif
 }
 
 // compile-time loop: f(std::integral_constant<int, i>) for i in [B, E) step S

@@ -10,89 +10,228 @@
 //
 // Layout of the work:
 //   * A frame of N real samples is one complex transform of M = N/2 points,
-//     z[n] = x[2n] + i x[2n+1] (loaded as float2, coalesced).  A group of
-//     T = M/16 threads owns it, 16 complex values per thread in VGPRs.
+//     z[n] = x[2n] + i x[2n+1].  A group of T = M/16 threads owns it, 16
+//     complex values per thread in VGPRs.
 //   * Stockham passes of radix 16 (plus one radix 2/4/8 pass) in registers;
 //     between passes the group exchanges through LDS (padded by one complex
 //     per 16).  The last pass leaves Z in LDS in natural order; the epilogue
 //     reads Z[k] and Z[M-k], forms the even/odd half spectra, applies the last
 //     radix-2 stage (or stops before it for the reference's odd-exponent
-//     result) and writes positions k and M+k of the frame, coalesced.
+//     result) and writes positions k and M+k of the frame.
 //   * Frames never share a transform, so a frame's result is bitwise the same
 //     whatever else is in the launch.
-//   * Workgroups of max(256, T) threads; blocks are renumbered so each XCD
-//     walks a contiguous range of frames (its own L2 sees the overlap halo).
+//   * A workgroup of max(256, T) threads owns G = WG/T consecutive frames.
+//     It stages their whole input span (overlap halo read once) in LDS with
+//     16-byte coalesced loads, and writes their whole output block with
+//     16-byte coalesced streaming stores; the FFT reuses the staging LDS.
+//   * Blocks are renumbered so each XCD walks a contiguous range of frames.
 #include "engine.hpp"
 #include "epilogue.hpp"
 #include "stockham.hpp"
 
 namespace ostft {
 
+// LDS float index of staged sample q: 4 pad floats after every 128, which
+// keeps 16-byte rows aligned for the vector writes.
+__device__ __forceinline__ uint32_t spad(uint32_t q) { return q + ((q >> 7) << 2); }
+
 template <int LOG_M>
 __global__ void __launch_bounds__(Cfg<LOG_M>::WG) stft_fused_kernel(const FusedArgs a)
 {
     using C = Cfg<LOG_M>;
-    constexpr int M = C::M, T = C::T, G = C::G, PADM = C::PADM;
+    constexpr int M = C::M, N = 2 * M, T = C::T, G = C::G, WG = C::WG, PADM = C::PADM;
     __shared__ float2 lds[G * PADM];
-    __shared__ float2 red[C::WG / 64];
+    __shared__ float2 red[WG / 64];
+    float* sf = reinterpret_cast<float*>(lds);
 
     const uint32_t tid = threadIdx.x;
     const uint32_t grp = tid / T, t = tid % T;
-    const uint64_t f = (uint64_t)xcd_block(blockIdx.x, gridDim.x) * G + grp;  // frame
-    const bool active = f < a.frames;
-    float2* z = lds + grp * PADM;
+    const uint64_t f0 = (uint64_t)xcd_block(blockIdx.x, gridDim.x) * G;  // first frame of the block
+    const uint32_t nf = (uint32_t)min((uint64_t)G, a.frames - f0);      // frames in this block
+    const bool active = grp < nf;
+    const uint64_t L = a.L, hop = a.hop;
     const float* __restrict__ in = a.in;
-    const uint64_t L = a.L;
 
-    // ---- gather + window (Overlap_Common + Window_*) ----
-    float2 v[16];
-    const uint64_t s0 = f * a.hop;
-    const bool aligned = ((s0 | (reinterpret_cast<uintptr_t>(in) >> 2)) & 1) == 0;
-    const bool inside = active && s0 + 2 * M <= L;
-    const float2* __restrict__ w2 = reinterpret_cast<const float2*>(a.win);
+    // ---- 1. issue every global load of the tile before the first wait ----
+    // (a) the block's input span (Overlap_Common): frames f0 .. f0+nf-1 read
+    //     samples [f0*hop, (f0+nf-1)*hop + N), loaded once with 16-byte loads
+    //     from a 4-sample-aligned origin, zero past L;
+    // (b) the window coefficients of this thread's 16 packed samples;
+    // (c) the base twiddles of every Stockham pass;
+    // (d) W_N^k for the epilogue bins of this thread.
+    const uint64_t s_begin = f0 * hop;
+    const uint64_t s_al = s_begin & ~3ull;
+    const uint64_t s_end = (f0 + nf - 1) * hop + N;
+    const uint32_t nvec = (uint32_t)((s_end - s_al + 3) >> 2);
+    const bool vec_ok = (reinterpret_cast<uintptr_t>(in) & 15u) == 0;
+    constexpr int MAXVEC = G * N / 4 + 2;
+    constexpr int ITER = (MAXVEC + WG - 1) / WG;
+    float4 x[ITER];
 #pragma unroll
-    for (int m = 0; m < 16; ++m) {
-        const uint32_t n = t + m * T;
-        const uint64_t s = s0 + 2u * n;
-        float2 x;
-        if (inside && aligned) {
-            x = *reinterpret_cast<const float2*>(in + s);
-        } else {
-            x.x = (active && s < L) ? in[s] : 0.0f;
-            x.y = (active && s + 1 < L) ? in[s + 1] : 0.0f;
+    for (int it = 0; it < ITER; ++it) {
+        const uint32_t i = tid + it * WG;
+        const uint64_t s = s_al + 4ull * i;
+        x[it] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (i < nvec) {
+            if (vec_ok && s + 4 <= L) {
+                x[it] = *reinterpret_cast<const float4*>(in + s);
+            } else {
+                x[it].x = s < L ? in[s] : 0.0f;
+                x[it].y = s + 1 < L ? in[s + 1] : 0.0f;
+                x[it].z = s + 2 < L ? in[s + 2] : 0.0f;
+                x[it].w = s + 3 < L ? in[s + 3] : 0.0f;
+            }
         }
-        if (w2) {
-            const float2 w = w2[n];
-            x = make_float2(x.x * w.x, x.y * w.y);
-        }
-        v[m] = x;
+    }
+    const float2* __restrict__ w2 = reinterpret_cast<const float2*>(a.win);
+    float2 wv[16];
+    if (w2) {
+#pragma unroll
+        for (int m = 0; m < 16; ++m)
+            wv[m] = w2[t + m * T];
+    }
+    PassTwiddles<LOG_M> ptw;
+    load_pass_twiddles<LOG_M>(ptw, a.tw, t);
+    // epilogue: the thread handles BPT consecutive bins per iteration, BPT = 4
+    // for the real layouts (one float4 per half) and 2 for the complex layout
+    // (one float4 of two complex values per half), so that every store
+    // instruction of a wave covers a contiguous 1 KB
+    const int mode = a.out_mode;
+    const uint32_t BPT = mode == OUT_HALF_COMPLEX ? 2u : 4u;
+    constexpr int OITER4 = (G * M) / (WG * 4);
+    constexpr int OITER2 = 2 * OITER4;
+    const int oiter = mode == OUT_HALF_COMPLEX ? OITER2 : OITER4;
+    const float2* __restrict__ twn = a.tw_n;
+    const bool vec_out = mode == OUT_COMPAT_POWER || mode == OUT_MAGNITUDE || mode == OUT_HALF_COMPLEX;
+    float2 eb[OITER2];  // W_N^k0 for each epilogue iteration
+    float2 ec1 = make_float2(1.0f, 0.0f), ec2 = ec1, ec3 = ec1;
+#pragma unroll
+    for (int it = 0; it < OITER2; ++it)
+        eb[it] = ec1;
+    if (vec_out && a.combine) {
+#pragma unroll
+        for (int it = 0; it < OITER2; ++it)
+            if (it < oiter)
+                eb[it] = twn[((tid + it * WG) * BPT) & (M - 1)];
+        ec1 = twn[1];
+        ec2 = twn[2];
+        ec3 = twn[3];
     }
 
-    // ---- DC removal of the windowed frame (DCRemove_Common) ----
+    // stage the span in LDS
+#pragma unroll
+    for (int it = 0; it < ITER; ++it) {
+        const uint32_t i = tid + it * WG;
+        if (i < nvec)
+            *reinterpret_cast<float4*>(sf + spad(4 * i)) = x[it];
+    }
+    __syncthreads();
+
+    // ---- 2. each group gathers its frame, packed z[n] = x[2n] + i x[2n+1],
+    //         and applies the window (Window_*) ----
+    float2 v[16];
+    {
+        const uint32_t q0 = (uint32_t)(grp * hop + (s_begin - s_al));
+#pragma unroll
+        for (int m = 0; m < 16; ++m) {
+            const uint32_t n = t + m * T;
+            float2 xv = make_float2(0.0f, 0.0f);
+            if (active) {
+                const uint32_t q = q0 + 2 * n;
+                xv = make_float2(sf[spad(q)], sf[spad(q + 1)]);
+            }
+            if (w2)
+                xv = make_float2(xv.x * wv[m].x, xv.y * wv[m].y);
+            v[m] = xv;
+        }
+    }
+    __syncthreads();  // the FFT reuses the staging buffer
+
+    // ---- 3. DC removal of the windowed frame (DCRemove_Common) ----
     if (a.dc) {
         float2 s = make_float2(0.0f, 0.0f);
 #pragma unroll
         for (int m = 0; m < 16; ++m)
             s = cadd(s, v[m]);
-        s = group_sum<T, C::WG>(s, red, tid);
-        const float mean = (s.x + s.y) / (float)(2 * M);
+        s = group_sum<T, WG>(s, red, tid);
+        const float mean = (s.x + s.y) / (float)N;
 #pragma unroll
         for (int m = 0; m < 16; ++m)
             v[m] = make_float2(v[m].x - mean, v[m].y - mean);
     }
 
-    // ---- FFT of the packed frame ----
-    stockham<LOG_M, 0>(v, z, a.tw, t);
+    // ---- 4. FFT of the packed frame; Z stays in LDS, natural order ----
+    stockham<LOG_M, 0>(v, lds + grp * PADM, ptw, t);
 
-    // ---- last radix-2 stage + output (toPower / toHalfComplexFormat) ----
-    if (!active)
-        return;
+    // ---- 5. last radix-2 stage + output, the whole block's output at once ----
+    float* __restrict__ out = a.out;
+    if (mode == OUT_HALF_COMPLEX) {
+        const uint32_t nbins = nf * (uint32_t)M;
+        const uint64_t lim = a.out_limit;  // complex values
+        float4* o = reinterpret_cast<float4*>(out);
 #pragma unroll
-    for (int i = 0; i < 16; ++i) {
-        const uint32_t k = t + i * T, km = (M - k) & (M - 1);
-        const float2 w = a.combine ? a.tw_n[k] : make_float2(1.0f, 0.0f);
-        const BinPair b = finish_bin(z[pad(k)], z[pad(km)], w, a.combine, false);
-        write_bin(a.out, a.out_mode, f, M, k, b, a.out_limit);
+        for (int it = 0; it < OITER2; ++it) {
+            const uint32_t e = (tid + it * WG) * 2;
+            if (e >= nbins)
+                break;
+            const uint32_t g = e >> LOG_M, k0 = e & (M - 1);
+            const float2* z = lds + g * PADM;
+            const BinPair b0 = finish_bin(z[pad(k0)], z[pad((M - k0) & (M - 1))], eb[it], a.combine, false);
+            const BinPair b1 = finish_bin(z[pad(k0 + 1)], z[pad(M - k0 - 1)], cmul(eb[it], ec1), a.combine, false);
+            const uint64_t base = (f0 + g) * (uint64_t)N;
+            if (base + k0 < lim)
+                nt_store4(make_float4(b0.lo.x, b0.lo.y, b1.lo.x, b1.lo.y), o + (base + k0) / 2);
+            if (base + M + k0 < lim)
+                nt_store4(make_float4(b0.hi.x, b0.hi.y, b1.hi.x, b1.hi.y), o + (base + M + k0) / 2);
+        }
+    } else if (vec_out) {
+        const uint32_t nbins = nf * (uint32_t)M;
+#pragma unroll
+        for (int it = 0; it < OITER4; ++it) {
+            const uint32_t e = (tid + it * WG) * 4;
+            if (e >= nbins)
+                break;
+            const uint32_t g = e >> LOG_M, k0 = e & (M - 1);
+            const float2* z = lds + g * PADM;
+            const float2 wk[4] = {eb[it], cmul(eb[it], ec1), cmul(eb[it], ec2), cmul(eb[it], ec3)};
+            BinPair b[4];
+#pragma unroll
+            for (uint32_t j = 0; j < 4; ++j) {
+                const uint32_t k = k0 + j, km = (M - k) & (M - 1);
+                b[j] = finish_bin(z[pad(k)], z[pad(km)], wk[j], a.combine, false);
+            }
+            const uint64_t base = (f0 + g) * (uint64_t)N;
+            float4 lo, hi;
+            if (mode == OUT_COMPAT_POWER) {
+                lo = make_float4(compat_power(b[0].lo), compat_power(b[1].lo), compat_power(b[2].lo),
+                                 compat_power(b[3].lo));
+                hi = make_float4(compat_power(b[0].hi), compat_power(b[1].hi), compat_power(b[2].hi),
+                                 compat_power(b[3].hi));
+            } else {
+                lo = make_float4(magnitude(b[0].lo), magnitude(b[1].lo), magnitude(b[2].lo), magnitude(b[3].lo));
+                hi = make_float4(magnitude(b[0].hi), magnitude(b[1].hi), magnitude(b[2].hi), magnitude(b[3].hi));
+            }
+            float4* o = reinterpret_cast<float4*>(out + base);
+            nt_store4(lo, o + k0 / 4);
+            nt_store4(hi, o + (M + k0) / 4);
+        }
+    } else {
+        // POWER / HALF_SPECTRUM: M+1 values per frame, one value per thread
+        const uint32_t nval = nf * (uint32_t)(M + 1);
+        for (uint32_t e = tid; e < nval; e += WG) {
+            const uint32_t g = e / (M + 1), k = e - g * (M + 1);
+            const float2* z = lds + g * PADM;
+            const uint32_t kb = k == (uint32_t)M ? 0u : k;
+            const uint32_t km = (M - kb) & (M - 1);
+            const float2 w = a.combine ? twn[kb] : make_float2(1.0f, 0.0f);
+            const BinPair b = finish_bin(z[pad(kb)], z[pad(km)], w, a.combine, false);
+            const float2 xv = k == (uint32_t)M ? b.hi : b.lo;
+            const uint64_t o = (f0 * (uint64_t)(M + 1)) + e;
+            if (mode == OUT_POWER)
+                nt_store(xv.x * xv.x + xv.y * xv.y, out + o);
+            else
+                nt_store(xv, reinterpret_cast<float2*>(out) + o);
+        }
     }
 }
 

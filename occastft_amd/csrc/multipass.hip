@@ -85,6 +85,8 @@ __global__ void __launch_bounds__(256) column_pass_kernel(const PassArgs a)
     const uint32_t g0 = (uint32_t)(blk - P * bpg) * CB;
     const uint32_t stride = (uint32_t)(M >> LOG_R);
     const uint32_t ns = 1u << a.log_ns;
+    PassTwiddles<LOG_R> tws;
+    load_pass_twiddles<LOG_R>(tws, a.tw_r, tid % T);
 
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
@@ -119,7 +121,7 @@ __global__ void __launch_bounds__(256) column_pass_kernel(const PassArgs a)
     for (int j = 0; j < 16; ++j)
         v[j] = z[pad(t + j * T)];
     __syncthreads();
-    stockham<LOG_R, 0>(v, z, a.tw_r, t);
+    stockham<LOG_R, 0>(v, z, tws, t);
 
     const uint64_t obase = P * M;
 #pragma unroll

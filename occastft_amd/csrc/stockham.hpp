@@ -12,13 +12,18 @@
 
 #include "fft_device.hpp"
 
+// smallest workgroup of the fused kernel (threads)
+#ifndef OSTFT_WG_MIN
+#define OSTFT_WG_MIN 256
+#endif
+
 namespace ostft {
 
 template <int LOG_M>
 struct Cfg {
     static constexpr int M = 1 << LOG_M;
     static constexpr int T = M / 16;               // threads per transform
-    static constexpr int WG = T < 256 ? 256 : T;   // threads per workgroup
+    static constexpr int WG = T < OSTFT_WG_MIN ? OSTFT_WG_MIN : T;  // threads per workgroup
     static constexpr int G = WG / T;               // transforms per workgroup
     static constexpr int PADM = M + M / 16;        // padded LDS length per transform
     static constexpr int NP16 = LOG_M / 4;
@@ -70,22 +75,72 @@ __device__ __forceinline__ float2 group_sum(float2 s, float2* red, uint32_t tid)
     return s;
 }
 
+// Base twiddles of every pass for one thread.  Pass p (radix r, stride
+// NS > 1) multiplies input m of butterfly g = t + u*T by w^m with
+// w = W_M^{(g mod NS) * M/(NS r)}; the thread keeps w (and w^4, w^8 for
+// radix 8/16) and forms the other powers by products.  The values depend only
+// on the thread's index, so they are loaded once, up front, with the tile's
+// other global loads.
+template <int LOG_M>
+struct PassTwiddles {
+    float2 b[Cfg<LOG_M>::NPASS][16][3];  // [pass][butterfly u][w, w^4, w^8]; unused slots vanish
+};
+
+template <int LOG_M>
+__device__ __forceinline__ void load_pass_twiddles(PassTwiddles<LOG_M>& P, const float2* __restrict__ tw,
+                                                   uint32_t t)
+{
+    using C = Cfg<LOG_M>;
+    constexpr int M = C::M, T = C::T;
+    cfor<1, C::NPASS>([&](auto p) {
+        constexpr int PP = decltype(p)::value;
+        constexpr int r = C::radix(PP), NS = C::ns(PP), U = 16 / r, STEP = M / (NS * r);
+        cfor<0, U>([&](auto u) {
+            constexpr int UU = decltype(u)::value;
+            const uint32_t e1 = ((t + UU * T) & (NS - 1)) * STEP;
+            P.b[PP][UU][0] = tw[e1];
+            if constexpr (r >= 8)
+                P.b[PP][UU][1] = tw[4 * e1];
+            if constexpr (r == 16)
+                P.b[PP][UU][2] = tw[8 * e1];
+        });
+    });
+}
+
 // Stockham pass PASS: twiddle, radix-r DFTs, scatter to LDS; then gather the
-// inputs of the next pass (or stop with the spectrum in LDS).
+// inputs of the next pass (or stop with the spectrum in LDS).  No global
+// memory is touched here.
 template <int LOG_M, int PASS>
-__device__ __forceinline__ void stockham(float2 (&v)[16], float2* z, const float2* __restrict__ tw, uint32_t t)
+__device__ __forceinline__ void stockham(float2 (&v)[16], float2* z, const PassTwiddles<LOG_M>& P, uint32_t t)
 {
     using C = Cfg<LOG_M>;
     constexpr int M = C::M, T = C::T;
     constexpr int r = C::radix(PASS), NS = C::ns(PASS), U = 16 / r;
     if constexpr (NS > 1) {
-        constexpr int STEP = M / (NS * r);
         cfor<0, U>([&](auto u) {
             constexpr int UU = decltype(u)::value;
-            const uint32_t kk = (t + UU * T) & (NS - 1);
+            float2 w[16];
+            w[1] = P.b[PASS][UU][0];
+            if constexpr (r >= 4) {
+                w[2] = cmul(w[1], w[1]);
+                w[3] = cmul(w[2], w[1]);
+            }
+            if constexpr (r >= 8) {
+                w[4] = P.b[PASS][UU][1];
+                w[5] = cmul(w[4], w[1]);
+                w[6] = cmul(w[4], w[2]);
+                w[7] = cmul(w[4], w[3]);
+            }
+            if constexpr (r == 16) {
+                w[8] = P.b[PASS][UU][2];
+                cfor<9, 16>([&](auto m) {
+                    constexpr int MM = decltype(m)::value;
+                    w[MM] = cmul(w[8], w[MM - 8]);
+                });
+            }
             cfor<1, r>([&](auto m) {
                 constexpr int MM = decltype(m)::value;
-                v[UU * r + MM] = cmul(v[UU * r + MM], tw[kk * (MM * STEP)]);
+                v[UU * r + MM] = cmul(v[UU * r + MM], w[MM]);
             });
         });
     }
@@ -114,7 +169,7 @@ __device__ __forceinline__ void stockham(float2 (&v)[16], float2* z, const float
             });
         });
         __syncthreads();
-        stockham<LOG_M, PASS + 1>(v, z, tw, t);
+        stockham<LOG_M, PASS + 1>(v, z, P, t);
     }
 }
 
