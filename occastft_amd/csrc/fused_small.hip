@@ -31,6 +31,13 @@
 
 namespace ostft {
 
+#ifndef OSTFT_SETPRIO
+#define OSTFT_SETPRIO 0
+#endif
+#ifndef OSTFT_PAIR_LDS
+#define OSTFT_PAIR_LDS 1
+#endif
+
 // LDS float index of staged sample q: 4 pad floats after every 128, which
 // keeps 16-byte rows aligned for the vector writes.
 __device__ __forceinline__ uint32_t spad(uint32_t q) { return q + ((q >> 7) << 2); }
@@ -53,6 +60,9 @@ __global__ void __launch_bounds__(Cfg<LOG_M>::WG) stft_fused_kernel(const FusedA
     const float* __restrict__ in = a.in;
 
     // ---- 1. issue every global load of the tile before the first wait ----
+#if OSTFT_SETPRIO
+    __builtin_amdgcn_s_setprio(3);
+#endif
     // (a) the block's input span (Overlap_Common): frames f0 .. f0+nf-1 read
     //     samples [f0*hop, (f0+nf-1)*hop + N), loaded once with 16-byte loads
     //     from a 4-sample-aligned origin, zero past L;
@@ -118,6 +128,9 @@ __global__ void __launch_bounds__(Cfg<LOG_M>::WG) stft_fused_kernel(const FusedA
         ec3 = twn[3];
     }
 
+#if OSTFT_SETPRIO
+    __builtin_amdgcn_s_setprio(0);
+#endif
     // stage the span in LDS
 #pragma unroll
     for (int it = 0; it < ITER; ++it) {
@@ -132,13 +145,18 @@ __global__ void __launch_bounds__(Cfg<LOG_M>::WG) stft_fused_kernel(const FusedA
     float2 v[16];
     {
         const uint32_t q0 = (uint32_t)(grp * hop + (s_begin - s_al));
+        // an even q0 keeps every (x[2n], x[2n+1]) pair inside one padded row
+        const bool pair_ok = (q0 & 1u) == 0;
 #pragma unroll
         for (int m = 0; m < 16; ++m) {
             const uint32_t n = t + m * T;
             float2 xv = make_float2(0.0f, 0.0f);
             if (active) {
                 const uint32_t q = q0 + 2 * n;
-                xv = make_float2(sf[spad(q)], sf[spad(q + 1)]);
+                if (OSTFT_PAIR_LDS && pair_ok)
+                    xv = *reinterpret_cast<const float2*>(sf + spad(q));
+                else
+                    xv = make_float2(sf[spad(q)], sf[spad(q + 1)]);
             }
             if (w2)
                 xv = make_float2(xv.x * wv[m].x, xv.y * wv[m].y);

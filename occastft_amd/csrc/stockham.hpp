@@ -12,9 +12,16 @@
 
 #include "fft_device.hpp"
 
-// smallest workgroup of the fused kernel (threads)
+// smallest workgroup of the fused kernel (threads), and from complex size
+// 2^OSTFT_BIG_LOG_M on
 #ifndef OSTFT_WG_MIN
 #define OSTFT_WG_MIN 256
+#endif
+#ifndef OSTFT_WG_BIG
+#define OSTFT_WG_BIG 256
+#endif
+#ifndef OSTFT_BIG_LOG_M
+#define OSTFT_BIG_LOG_M 99
 #endif
 
 namespace ostft {
@@ -23,7 +30,8 @@ template <int LOG_M>
 struct Cfg {
     static constexpr int M = 1 << LOG_M;
     static constexpr int T = M / 16;               // threads per transform
-    static constexpr int WG = T < OSTFT_WG_MIN ? OSTFT_WG_MIN : T;  // threads per workgroup
+    static constexpr int WG_MIN = LOG_M >= OSTFT_BIG_LOG_M ? OSTFT_WG_BIG : OSTFT_WG_MIN;
+    static constexpr int WG = T < WG_MIN ? WG_MIN : T;  // threads per workgroup
     static constexpr int G = WG / T;               // transforms per workgroup
     static constexpr int PADM = M + M / 16;        // padded LDS length per transform
     static constexpr int NP16 = LOG_M / 4;
