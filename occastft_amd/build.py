@@ -66,6 +66,27 @@ def build(verbose: bool = False, out: str = LIB, build_dir: str = BUILD, defines
     return out
 
 
+RUNNER_SOURCES = ["runner/hip_run.cpp", "runner/request_codec.cpp", "runner/websocket.cpp"]
+RUNNER = os.path.join(HERE, "hipRun")
+
+
+def build_runner(verbose: bool = False) -> str:
+    """Link the `hipRun` runner executable (host C++) against the engine."""
+    cxx = shutil.which("g++") or "g++"
+    tmp = RUNNER + ".tmp"
+    cmd = [cxx, "-O2", "-std=c++17", "-pthread", "-Wall", "-I", os.path.join(HERE, "..", "include"),
+           *[os.path.join(CSRC, s) for s in RUNNER_SOURCES], "-L", HERE, "-locca_stft_hip",
+           "-Wl,-rpath,$ORIGIN", "-Wl,-rpath-link,/opt/rocm/lib", "-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipRun build failed:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, RUNNER)
+    if verbose:
+        print(f"built {RUNNER}")
+    return RUNNER
+
+
 if __name__ == "__main__":
     build(verbose=True)
+    build_runner(verbose=True)
     sys.exit(0)

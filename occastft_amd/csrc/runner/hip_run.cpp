@@ -1,0 +1,304 @@
+// hip_run.cpp -- `hipRun`, the MI355X runner process.
+//
+// A drop-in for the reference runner executables (serialRun, openmpRun,
+// cudaRun): `hipRun [port]` listens on 0.0.0.0:port (default 54500) for
+// WebSocket connections and answers Cap'n Proto RequestCapnp messages, with
+// the input either in the message's data list or in the POSIX shared memory
+// object the request names (RunnerCommon.cpp:61-164, FFTStruct_linux.cpp).
+// The reply copies the request's fields (without options), carries the result
+// in the data list or writes it into the shared memory, and marks a failed
+// request with sharedMemory = "ERR" (FFTStruct.cpp:15-22).  A text message
+// "CLOSE_REQUEST" is answered with "CLOSE_COMPLETE" and stops the server.
+//
+// Differences from the reference runners, on purpose: every request keeps its
+// own shared-memory mapping (the reference shares one `dataInfo` member across
+// connection threads, RunnerInterface.hpp:120); an undecodable message is
+// answered with an "ERR" reply instead of an exception; a result larger than
+// the client's mapping is an error instead of an overflow.
+#include <fcntl.h>
+#include <sys/mman.h>
+#include <unistd.h>
+
+#include <cstdio>
+#include <cstring>
+#include <iostream>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../../include/occastft_hip.h"
+#include "request_codec.hpp"
+#include "websocket.hpp"
+
+namespace {
+
+using ostft_wire::Request;
+
+// page-rounded byte size of n floats (FFTStruct.hpp:138-145)
+uint64_t page_bytes(uint64_t nfloats)
+{
+    const uint64_t page = (uint64_t)sysconf(_SC_PAGESIZE);
+    const uint64_t bytes = nfloats * sizeof(float);
+    return ((bytes + page - 1) / page) * page;
+}
+
+struct ShmMap {
+    void* ptr = nullptr;
+    int fd = -1;
+    uint64_t bytes = 0;
+    ~ShmMap()
+    {
+        if (ptr && ptr != MAP_FAILED)
+            munmap(ptr, bytes);
+        if (fd >= 0)
+            close(fd);
+    }
+};
+
+// The runner: the reference's Runner shape (RunnerInterface.hpp:108-131) with
+// the HIP engine behind ActivateSTFT.
+class HipRunner {
+public:
+    explicit HipRunner(int port) : server_(port)
+    {
+        InitEnv();
+        BuildKernel();
+    }
+    ~HipRunner() { UnInit(); }
+
+    bool ServerInit(std::string& err) { return server_.listen(err); }
+    void Serve()
+    {
+        server_.serve([this](ostft_wire::WsConnection& c, int op, std::string& msg) { OnMessage(c, op, msg); });
+    }
+
+    // Runner::ActivateSTFT: returns false where the reference returns nullopt.
+    bool ActivateSTFT(const std::vector<float>& in, int exp, float overlap, const std::string& options,
+                      std::vector<float>& out, std::string& why)
+    {
+        std::lock_guard<std::mutex> lk(engine_mu_);  // one context, calls must not overlap
+        if (!ctx_) {
+            why = "no HIP device";
+            return false;
+        }
+        ostft_params p;
+        int rc = ostft_parse_options(options.c_str(), exp, overlap, &p);
+        ostft_geometry g;
+        if (rc == OSTFT_OK)
+            rc = ostft_get_geometry(&p, in.size(), &g);
+        if (rc != OSTFT_OK) {
+            why = ostft_strerror(rc);
+            return false;
+        }
+        out.resize(g.out_floats);
+        uint64_t n = 0;
+        rc = ostft_activate(ctx_, in.empty() ? nullptr : in.data(), in.size(), exp, overlap, options.c_str(),
+                            out.data(), out.size(), &n);
+        if (rc != OSTFT_OK) {
+            why = std::string(ostft_strerror(rc)) + ": " + ostft_last_error(ctx_);
+            return false;
+        }
+        out.resize(n);
+        return true;
+    }
+
+private:
+    void InitEnv()
+    {
+        if (ostft_create(0, &ctx_) != OSTFT_OK) {
+            ctx_ = nullptr;
+            std::cerr << "hipRun: no HIP device; every request will be answered with ERR" << std::endl;
+        }
+    }
+    void BuildKernel() {}  // kernels are compiled ahead of time into the engine library
+    void UnInit()
+    {
+        if (ctx_)
+            ostft_destroy(ctx_);
+        ctx_ = nullptr;
+    }
+
+    // RunnerCommon.cpp AccessData: the input from shm or from the data list.
+    bool AccessData(const Request& req, std::vector<float>& data, ShmMap& shm, std::string& why)
+    {
+        if (!req.sharedMemory.empty()) {
+            shm.fd = shm_open(req.sharedMemory.c_str(), O_RDWR, 0666);
+            if (shm.fd < 0) {
+                why = "shm_open failed";
+                return false;
+            }
+            shm.bytes = page_bytes(req.overlapdataLength);
+            if (shm.bytes == 0 || req.dataLength * sizeof(float) > shm.bytes) {
+                why = "shared memory smaller than the input";
+                return false;
+            }
+            shm.ptr = mmap(nullptr, shm.bytes, PROT_READ | PROT_WRITE, MAP_SHARED, shm.fd, 0);
+            if (shm.ptr == MAP_FAILED) {
+                shm.ptr = nullptr;
+                why = "mmap failed";
+                return false;
+            }
+            data.resize(req.dataLength);
+            std::memcpy(data.data(), shm.ptr, req.dataLength * sizeof(float));
+            return true;
+        }
+        if (req.data.empty()) {
+            why = "no input data";
+            return false;
+        }
+        data = req.data;
+        return true;
+    }
+
+    void OnMessage(ostft_wire::WsConnection& conn, int opcode, std::string& msg)
+    {
+        if (opcode == 1) {
+            if (msg == "CLOSE_REQUEST") {
+                conn.send_text("CLOSE_COMPLETE");
+                std::thread([this] { server_.stop(); }).detach();
+            }
+            return;
+        }
+        Request req, rep;
+        std::string err;
+        const bool decoded
+            = ostft_wire::decode_request(reinterpret_cast<const uint8_t*>(msg.data()), msg.size(), req, err);
+        msg.clear();
+        msg.shrink_to_fit();
+        std::vector<float> result;
+        bool ok = decoded;
+        if (decoded) {
+            // FFTRequest::MakeWField: every field but options and data
+            rep.windowRadix = req.windowRadix;
+            rep.overlapRatio = req.overlapRatio;
+            rep.mappedID = req.mappedID;
+            rep.sharedMemory = req.sharedMemory;
+            rep.memPTR = req.memPTR;
+            rep.posixFileDes = req.posixFileDes;
+            rep.windowsHandlePTR = req.windowsHandlePTR;
+            rep.dataLength = req.dataLength;
+            rep.overlapdataLength = req.overlapdataLength;
+            ShmMap shm;
+            std::vector<float> data;
+            ok = AccessData(req, data, shm, err);
+            req.data.clear();
+            req.data.shrink_to_fit();
+            if (ok)
+                ok = ActivateSTFT(data, (int)req.windowRadix, req.overlapRatio, req.options, result, err);
+            if (ok && shm.ptr) {
+                if (result.size() * sizeof(float) > shm.bytes) {
+                    ok = false;
+                    err = "result larger than the shared memory";
+                } else {
+                    std::memcpy(shm.ptr, result.data(), result.size() * sizeof(float));
+                    result.clear();
+                }
+            }
+        }
+        if (!ok) {
+            std::cerr << "hipRun: request " << rep.mappedID << " failed: " << err << std::endl;
+            rep.sharedMemory = "ERR";
+            result.clear();
+        } else if (!result.empty()) {
+            rep.dataLength = result.size();  // FFTRequest::SetData
+        }
+        std::string out;
+        if (!ostft_wire::encode_request(rep, result.data(), result.size(), out)) {
+            rep.sharedMemory = "ERR";
+            ostft_wire::encode_request(rep, nullptr, 0, out);
+        }
+        conn.send_binary(out);
+    }
+
+    ostft_wire::WsServer server_;
+    ostft_ctx* ctx_ = nullptr;
+    std::mutex engine_mu_;
+};
+
+}  // namespace
+
+// `hipRun --decode FILE [OUT]`: print a request message's fields as JSON
+// (and, with OUT, write it back re-encoded as one segment) -- a diagnostic for
+// client authors.
+int decode_file(const char* path, const char* out_path)
+{
+    FILE* fh = std::fopen(path, "rb");
+    if (!fh) {
+        std::perror(path);
+        return 1;
+    }
+    std::string buf;
+    char chunk[1 << 16];
+    size_t r;
+    while ((r = std::fread(chunk, 1, sizeof(chunk), fh)) > 0)
+        buf.append(chunk, r);
+    std::fclose(fh);
+    Request q;
+    std::string err;
+    if (!ostft_wire::decode_request(reinterpret_cast<const uint8_t*>(buf.data()), buf.size(), q, err)) {
+        std::printf("{\"ok\": false, \"error\": \"%s\"}\n", err.c_str());
+        return 1;
+    }
+    double sum = 0.0;
+    for (float v : q.data)
+        sum += v;
+    auto js = [](const std::string& s) {
+        std::string o;
+        for (char c : s) {
+            if (c == '"' || c == '\\')
+                o += '\\';
+            if ((unsigned char)c < 0x20) {
+                char b[8];
+                std::snprintf(b, sizeof(b), "\\u%04x", (unsigned char)c);
+                o += b;
+                continue;
+            }
+            o += c;
+        }
+        return o;
+    };
+    std::printf("{\"ok\": true, \"sharedMemory\": \"%s\", \"mappedID\": \"%s\", \"options\": \"%s\", "
+                "\"memPTR\": %llu, \"posixFileDes\": %lld, \"windowsHandlePTR\": %llu, \"windowRadix\": %u, "
+                "\"overlapRatio\": %.9g, \"dataLength\": %llu, \"overlapdataLength\": %llu, "
+                "\"data_count\": %zu, \"data_sum\": %.17g}\n",
+                js(q.sharedMemory).c_str(), js(q.mappedID).c_str(), js(q.options).c_str(),
+                (unsigned long long)q.memPTR, (long long)q.posixFileDes, (unsigned long long)q.windowsHandlePTR,
+                q.windowRadix, (double)q.overlapRatio, (unsigned long long)q.dataLength,
+                (unsigned long long)q.overlapdataLength, q.data.size(), sum);
+    if (out_path) {
+        std::string out;
+        if (!ostft_wire::encode_request(q, q.data.data(), q.data.size(), out))
+            return 1;
+        FILE* oh = std::fopen(out_path, "wb");
+        if (!oh || std::fwrite(out.data(), 1, out.size(), oh) != out.size())
+            return 1;
+        std::fclose(oh);
+    }
+    return 0;
+}
+
+int main(int argc, char* argv[])
+{
+    if (argc >= 3 && std::strcmp(argv[1], "--decode") == 0)
+        return decode_file(argv[2], argc >= 4 ? argv[3] : nullptr);
+    int port = 54500;  // RunnerCommon.cpp:146
+    if (argc == 2) {
+        try {
+            port = std::stoi(argv[1]);
+        } catch (const std::exception& e) {
+            std::cerr << e.what() << '\n';
+            return 1;
+        }
+    }
+    HipRunner runner(port);
+    std::string err;
+    if (!runner.ServerInit(err)) {
+        std::cerr << "ERR on server init: " << err << std::endl;
+        return 1;
+    }
+    std::printf("hipRun listening on %d\n", port);
+    std::fflush(stdout);
+    runner.Serve();
+    return 0;
+}
