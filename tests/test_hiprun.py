@@ -219,6 +219,9 @@ def test_concurrent_clients(runner):
         try:
             c = runner.client()
             for _ in range(3):
+                # the result overwrites the input in the mapping, as in the
+                # reference protocol: put the input back before each request
+                s.write(x)
                 c.send_binary(wc.encode(s.req))
                 c.recv_message()
             n = po.geometry(len(x), 9 + i, 0.5)["ofull"]

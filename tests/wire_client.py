@@ -319,6 +319,9 @@ class ShmRequest:
                        overlapRatio=ov, dataLength=len(x), overlapdataLength=olen)
         self.olen = olen
 
+    def write(self, x: np.ndarray):
+        np.ndarray(len(x), np.float32, buffer=self.shm.buf)[:] = x
+
     def result(self) -> np.ndarray:
         return np.ndarray(self.olen, np.float32, buffer=self.shm.buf).copy()
 
