@@ -42,6 +42,8 @@ extern "C" {
 #define OSTFT_ERR_HIP 5            /* a HIP runtime call failed; see ostft_last_error */
 #define OSTFT_ERR_NO_DEVICE 6      /* device index out of range / no GPU */
 #define OSTFT_ERR_TOO_LARGE 7      /* sizes beyond what the engine addresses */
+#define OSTFT_ERR_TIMEOUT 8        /* ostft_server_wait: the request is still running */
+#define OSTFT_ERR_SHUTDOWN 9       /* ostft_server_submit after shutdown began */
 
 /* window kinds, in Default_Pipeline's precedence order (RunnerCommon.cpp:216-247) */
 enum ostft_window {
@@ -136,6 +138,93 @@ int ostft_run_device(ostft_ctx* ctx, const ostft_params* params, const float* d_
 int ostft_activate(ostft_ctx* ctx, const float* in, uint64_t full_size, int32_t exp,
                    float overlap, const char* options, float* out, uint64_t out_cap,
                    uint64_t* out_len);
+
+/* ---- batches: many requests in one launch sequence ----
+ *
+ * No counterpart in the reference, which runs one request per
+ * Runner::ActivateSTFT (StandAlone/cross_gpgpu/Serial/src/SerialImpl.cpp:30-75);
+ * each request here has exactly the semantics of ostft_run_device. */
+typedef struct ostft_job {
+    const float* in;       /* device pointer, full_size floats */
+    uint64_t full_size;
+    float* out;            /* device pointer, 16-byte aligned, out_cap floats */
+    uint64_t out_cap;
+    ostft_params params;
+} ostft_job;
+
+/* Runs `njobs` device-resident requests of any window size, overlap, window
+ * and output layout: one persistent launch per window size present (windows
+ * 2^6..2^14), whose workgroups pull tiles of frames from a device-side
+ * counter; larger windows run after them, request by request.  Every
+ * request's result is bitwise identical to running it alone with
+ * ostft_run_device.  All requests are validated before anything is launched;
+ * a bad one fails the call (its index is in ostft_last_error).  Queued on
+ * `hip_stream`, no wait.  Outputs must not overlap. */
+int ostft_run_batch_device(ostft_ctx* ctx, const ostft_job* jobs, uint32_t njobs, void* hip_stream);
+
+/* Batch plan of window exponent `exp` (6..14): blocks_per_cu > 0 sets the
+ * persistent workgroups per CU (clamped to the occupancy limit); use_graphs
+ * 0/1 turns replay of captured launch sequences off/on (-1 leaves it; exp 0
+ * sets only that); *current receives the setting in force. */
+int ostft_batch_config(ostft_ctx* ctx, int32_t exp, int32_t blocks_per_cu, int32_t use_graphs,
+                       int32_t* current);
+
+/* Times the persistent launch of window exponent `exp` on a synthetic ragged
+ * batch of `samples` samples for every workgroups-per-CU setting, keeps the
+ * fastest for this context and reports it. */
+int ostft_autotune_batch(ostft_ctx* ctx, int32_t exp, uint64_t samples, int32_t* best_blocks_per_cu,
+                         double* best_ms);
+
+/* ---- serving: continuous batching of host-buffer requests ----
+ *
+ * No counterpart in the reference, whose runner serves each WebSocket request
+ * synchronously on its connection thread (StandAlone/cross_gpgpu/GLOBAL/
+ * RunnerCommon.cpp:77-120).  A server owns a context, a device arena and a
+ * pinned host mirror of it.  Submitted requests wait in FIFO order; a
+ * scheduler thread admits as many as fit in the arena (by their input +
+ * output footprint) into the next batch while earlier batches are still on
+ * the GPU, copies them in, runs them with ostft_run_batch_device and copies
+ * the results out; a completion thread hands results to the callers.  Up to
+ * max_inflight batches overlap (copies in, kernels, copies out on separate
+ * streams).  All functions are thread-safe. */
+typedef struct ostft_server ostft_server;
+
+typedef struct ostft_server_config {
+    uint64_t arena_bytes;   /* device arena for admitted requests; 0: 1 GiB */
+    uint32_t max_batch;     /* requests per batch; 0: 512 */
+    uint32_t max_inflight;  /* batches on the GPU at once; 0: 3 */
+    int32_t use_graphs;     /* 1 (default when the config is NULL): replay captured launches */
+    int32_t autotune;       /* 1: time the persistent grid of every window size at creation */
+} ostft_server_config;
+
+typedef struct ostft_server_stats {
+    uint64_t submitted, completed, failed, batches;
+    uint64_t frames;          /* frames transformed by completed requests */
+    uint64_t bytes_in, bytes_out;
+    double latency_p50_ms;    /* submit -> result in the caller's buffer, completed requests */
+    double latency_p99_ms;
+    double latency_max_ms;
+    double latency_mean_ms;
+    double mean_batch;        /* requests per batch */
+    double span_s;            /* first submit -> last completion */
+} ostft_server_stats;
+
+int ostft_server_create(int32_t device, const ostft_server_config* config, ostft_server** out);
+/* Finishes every submitted request, then releases everything. */
+void ostft_server_destroy(ostft_server* srv);
+/* Queues one request.  `in` and `out` are host memory that must stay valid
+ * until ostft_server_wait returns for the ticket.  Parameters, capacity and
+ * footprint are checked here (the request's own error code is returned and no
+ * ticket is issued). */
+int ostft_server_submit(ostft_server* srv, const float* in, uint64_t full_size, const ostft_params* params,
+                        float* out, uint64_t out_cap, uint64_t* ticket);
+/* Waits up to timeout_ms (< 0: no limit) for a request; returns its status
+ * (OSTFT_ERR_TIMEOUT if it is not done; the ticket stays valid then) and the
+ * floats written.  A ticket is released once its status has been returned. */
+int ostft_server_wait(ostft_server* srv, uint64_t ticket, int32_t timeout_ms, uint64_t* out_len);
+/* Counters and latency percentiles since creation (or the last reset). */
+int ostft_server_stats_get(ostft_server* srv, ostft_server_stats* out, int32_t reset);
+const char* ostft_server_last_error(const ostft_server* srv);
 
 /* Host-side window coefficients the engine uses by default for `kind`, size n:
  * the reference's own float/double evaluation (OKL/kernel.okl:33-127 as

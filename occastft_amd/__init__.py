@@ -27,10 +27,11 @@ from ._native import (  # noqa: F401
     NativeMissing,
     Params,
 )
-from .runner import Runner, STFTError, geometry, make_params, parse_options  # noqa: F401
+from .runner import Runner, Server, STFTError, geometry, make_params, parse_options  # noqa: F401
 
 __all__ = [
     "Runner",
+    "Server",
     "STFTError",
     "NativeMissing",
     "Params",

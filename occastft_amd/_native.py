@@ -20,6 +20,8 @@ ERR_CAPACITY = 4
 ERR_HIP = 5
 ERR_NO_DEVICE = 6
 ERR_TOO_LARGE = 7
+ERR_TIMEOUT = 8
+ERR_SHUTDOWN = 9
 
 # windows
 WIN_NONE, WIN_HANNING, WIN_HAMMING, WIN_BLACKMAN, WIN_NUTTALL = 0, 1, 2, 3, 4
@@ -46,6 +48,15 @@ EXPORTS = (
     "ostft_last_error",
     "ostft_abi_version",
     "ostft_window_coefficients",
+    "ostft_run_batch_device",
+    "ostft_batch_config",
+    "ostft_autotune_batch",
+    "ostft_server_create",
+    "ostft_server_destroy",
+    "ostft_server_submit",
+    "ostft_server_wait",
+    "ostft_server_stats_get",
+    "ostft_server_last_error",
 )
 
 
@@ -83,6 +94,47 @@ class Geometry(ctypes.Structure):
         return {f: int(getattr(self, f)) for f, _ in self._fields_}
 
 
+class Job(ctypes.Structure):
+    _fields_ = [
+        ("in_ptr", ctypes.c_void_p),
+        ("full_size", ctypes.c_uint64),
+        ("out_ptr", ctypes.c_void_p),
+        ("out_cap", ctypes.c_uint64),
+        ("params", Params),
+    ]
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [
+        ("arena_bytes", ctypes.c_uint64),
+        ("max_batch", ctypes.c_uint32),
+        ("max_inflight", ctypes.c_uint32),
+        ("use_graphs", ctypes.c_int32),
+        ("autotune", ctypes.c_int32),
+    ]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [
+        ("submitted", ctypes.c_uint64),
+        ("completed", ctypes.c_uint64),
+        ("failed", ctypes.c_uint64),
+        ("batches", ctypes.c_uint64),
+        ("frames", ctypes.c_uint64),
+        ("bytes_in", ctypes.c_uint64),
+        ("bytes_out", ctypes.c_uint64),
+        ("latency_p50_ms", ctypes.c_double),
+        ("latency_p99_ms", ctypes.c_double),
+        ("latency_max_ms", ctypes.c_double),
+        ("latency_mean_ms", ctypes.c_double),
+        ("mean_batch", ctypes.c_double),
+        ("span_s", ctypes.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 _lib = None
 
 
@@ -117,6 +169,26 @@ def lib() -> ctypes.CDLL:
     L.ostft_abi_version.argtypes = []
     L.ostft_window_coefficients.restype = ctypes.c_int
     L.ostft_window_coefficients.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_float, fp]
+    i32p = ctypes.POINTER(ctypes.c_int32)
+    u64p = ctypes.POINTER(ctypes.c_uint64)
+    L.ostft_run_batch_device.restype = ctypes.c_int
+    L.ostft_run_batch_device.argtypes = [vp, ctypes.POINTER(Job), ctypes.c_uint32, vp]
+    L.ostft_batch_config.restype = ctypes.c_int
+    L.ostft_batch_config.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, i32p]
+    L.ostft_autotune_batch.restype = ctypes.c_int
+    L.ostft_autotune_batch.argtypes = [vp, ctypes.c_int32, ctypes.c_uint64, i32p, ctypes.POINTER(ctypes.c_double)]
+    L.ostft_server_create.restype = ctypes.c_int
+    L.ostft_server_create.argtypes = [ctypes.c_int32, ctypes.POINTER(ServerConfig), ctypes.POINTER(vp)]
+    L.ostft_server_destroy.restype = None
+    L.ostft_server_destroy.argtypes = [vp]
+    L.ostft_server_submit.restype = ctypes.c_int
+    L.ostft_server_submit.argtypes = [vp, vp, ctypes.c_uint64, ctypes.POINTER(Params), vp, ctypes.c_uint64, u64p]
+    L.ostft_server_wait.restype = ctypes.c_int
+    L.ostft_server_wait.argtypes = [vp, ctypes.c_uint64, ctypes.c_int32, u64p]
+    L.ostft_server_stats_get.restype = ctypes.c_int
+    L.ostft_server_stats_get.argtypes = [vp, ctypes.POINTER(ServerStats), ctypes.c_int32]
+    L.ostft_server_last_error.restype = ctypes.c_char_p
+    L.ostft_server_last_error.argtypes = [vp]
     _lib = L
     return L
 

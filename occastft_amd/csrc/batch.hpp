@@ -1,0 +1,29 @@
+// batch.hpp -- many requests in one launch per window-size class (batch.hip).
+#pragma once
+
+#include <string>
+
+#include "../../include/occastft_hip.h"
+#include "engine.hpp"
+
+struct ostft_ctx;
+
+namespace ostft {
+
+struct BatchState;
+
+// accessors into the context (defined in capi.hip)
+BatchState*& ctx_batch(ostft_ctx* ctx);
+int ctx_device(const ostft_ctx* ctx);
+int ctx_fail(ostft_ctx* ctx, int code, const std::string& msg);
+int ctx_hip_fail(ostft_ctx* ctx, hipError_t e, const char* where);
+int ctx_prepare(ostft_ctx* ctx, const ostft_params* params, const float* d_in, uint64_t full_size, float* d_out,
+                uint64_t out_cap, FusedArgs* a, int* log_m);
+
+// The batch entry points behind the C ABI (the context's device is current).
+int run_batch(ostft_ctx* ctx, const ostft_job* jobs, uint32_t njobs, hipStream_t s);
+int batch_config(ostft_ctx* ctx, int exp, int blocks_per_cu, int use_graphs, int* current);
+int autotune_batch(ostft_ctx* ctx, int exp, uint64_t samples, int* best_blocks_per_cu, double* best_ms);
+void batch_release(ostft_ctx* ctx);
+
+}  // namespace ostft

@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "engine.hpp"
+#include "batch.hpp"
 #include "multipass.hpp"
 
 struct ostft_ctx {
@@ -31,6 +32,7 @@ struct ostft_ctx {
     float* d_out = nullptr;
     uint64_t d_out_cap = 0;
     ostft::Scratch scratch;
+    ostft::BatchState* batch = nullptr;  // batch descriptors, graphs, plans (batch.hip)
     std::string last_error;
 };
 
@@ -246,9 +248,64 @@ float* get_window(ostft_ctx* ctx, int kind, int64_t n, float sigma, int exact, i
 }  // namespace
 
 namespace ostft {
-// used by multipass.hip
+// used by multipass.hip and batch.hip
 float2* ctx_twiddles(ostft_ctx* ctx, int64_t m, int64_t count, int* rc) { return get_twiddles(ctx, m, count, rc); }
 Scratch& ctx_scratch(ostft_ctx* ctx) { return ctx->scratch; }
+BatchState*& ctx_batch(ostft_ctx* ctx) { return ctx->batch; }
+int ctx_device(const ostft_ctx* ctx) { return ctx->device; }
+int ctx_fail(ostft_ctx* ctx, int code, const std::string& msg) { return fail(ctx, code, msg); }
+int ctx_hip_fail(ostft_ctx* ctx, hipError_t e, const char* where) { return hip_fail(ctx, e, where); }
+
+// Validates one device-resident request and resolves its tables (window,
+// twiddles): everything a launch needs, in `a`.  The caller's device is current.
+int ctx_prepare(ostft_ctx* ctx, const ostft_params* params, const float* d_in, uint64_t full_size, float* d_out,
+                uint64_t out_cap, FusedArgs* a, int* log_m)
+{
+    if (!params || !d_out || (!d_in && full_size > 0))
+        return fail(ctx, OSTFT_ERR_INVALID_ARG, "null argument");
+    Plan plan;
+    int rc = make_plan(params, full_size, &plan, ctx);
+    if (rc != OSTFT_OK)
+        return rc;
+    if (out_cap < plan.g.out_floats)
+        return fail(ctx, OSTFT_ERR_CAPACITY, "output buffer holds " + std::to_string(out_cap) + " floats, need "
+                                                 + std::to_string(plan.g.out_floats));
+    if ((reinterpret_cast<uintptr_t>(d_out) & 15u) != 0)
+        return fail(ctx, OSTFT_ERR_INVALID_ARG, "device output must be 16-byte aligned");
+    const uint64_t N = plan.g.window_size;
+    int win_kind = params->window;
+    if (win_kind == OSTFT_WIN_GAUSSIAN && !(params->sigma > 0.0f))
+        win_kind = OSTFT_WIN_NONE;  // the reference skips the window for sigma <= 0
+    const float* w = nullptr;
+    if (win_kind != OSTFT_WIN_NONE) {
+        w = get_window(ctx, win_kind, (int64_t)N, params->sigma, (params->flags & OSTFT_FLAG_EXACT_WINDOW) ? 1 : 0,
+                       &rc);
+        if (!w)
+            return rc;
+    }
+    const int64_t M = 1ll << plan.log_m;
+    a->in = d_in;
+    a->L = full_size;
+    a->hop = plan.g.hop;
+    a->frames = plan.g.compute_frames;
+    a->win = w;
+    a->out = d_out;
+    a->out_limit = plan.g.out_half;
+    a->combine = plan.combine;
+    a->out_mode = params->output;
+    a->dc = (params->flags & OSTFT_FLAG_REMOVE_DC) ? 1 : 0;
+    a->tw = nullptr;
+    a->tw_n = get_twiddles(ctx, 2 * M, M, &rc);  // W_N^k, k < M
+    if (!a->tw_n)
+        return rc;
+    if (fused_supported(plan.log_m)) {
+        a->tw = get_twiddles(ctx, M, M, &rc);
+        if (!a->tw)
+            return rc;
+    }
+    *log_m = plan.log_m;
+    return OSTFT_OK;
+}
 }  // namespace ostft
 
 extern "C" {
@@ -266,6 +323,8 @@ const char* ostft_strerror(int status)
     case OSTFT_ERR_HIP: return "HIP runtime error";
     case OSTFT_ERR_NO_DEVICE: return "no such HIP device";
     case OSTFT_ERR_TOO_LARGE: return "request too large";
+    case OSTFT_ERR_TIMEOUT: return "request still running";
+    case OSTFT_ERR_SHUTDOWN: return "server is shutting down";
     default: return "unknown status";
     }
 }
@@ -398,6 +457,7 @@ void ostft_destroy(ostft_ctx* ctx)
         if (ctx->d_out)
             (void)hipFree(ctx->d_out);
         ctx->scratch.release();
+        ostft::batch_release(ctx);
         if (ctx->stream)
             (void)hipStreamDestroy(ctx->stream);
     }
@@ -407,63 +467,57 @@ void ostft_destroy(ostft_ctx* ctx)
 int ostft_run_device(ostft_ctx* ctx, const ostft_params* params, const float* d_in, uint64_t full_size,
                      float* d_out, uint64_t out_cap, void* hip_stream)
 {
-    if (!ctx || !params || !d_out || (!d_in && full_size > 0))
-        return fail(ctx, OSTFT_ERR_INVALID_ARG, "null argument");
-    Plan plan;
-    int rc = make_plan(params, full_size, &plan, ctx);
-    if (rc != OSTFT_OK)
-        return rc;
-    if (out_cap < plan.g.out_floats)
-        return fail(ctx, OSTFT_ERR_CAPACITY, "output buffer holds " + std::to_string(out_cap) + " floats, need "
-                                                 + std::to_string(plan.g.out_floats));
-    if ((reinterpret_cast<uintptr_t>(d_out) & 15u) != 0)
-        return fail(ctx, OSTFT_ERR_INVALID_ARG, "device output must be 16-byte aligned");
+    if (!ctx)
+        return OSTFT_ERR_INVALID_ARG;
     DeviceGuard dg(ctx->device);
     if (!dg.ok)
         return fail(ctx, OSTFT_ERR_NO_DEVICE, "hipSetDevice failed");
-    hipStream_t s = static_cast<hipStream_t>(hip_stream);
-    const uint64_t N = plan.g.window_size;
-
-    int win_kind = params->window;
-    if (win_kind == OSTFT_WIN_GAUSSIAN && !(params->sigma > 0.0f))
-        win_kind = OSTFT_WIN_NONE;  // the reference skips the window for sigma <= 0
-    const float* w = nullptr;
-    if (win_kind != OSTFT_WIN_NONE) {
-        w = get_window(ctx, win_kind, (int64_t)N, params->sigma, (params->flags & OSTFT_FLAG_EXACT_WINDOW) ? 1 : 0,
-                       &rc);
-        if (!w)
-            return rc;
-    }
-    const int64_t M = 1ll << plan.log_m;
-
     ostft::FusedArgs a;
-    a.in = d_in;
-    a.L = full_size;
-    a.hop = plan.g.hop;
-    a.frames = plan.g.compute_frames;
-    a.win = w;
-    a.out = d_out;
-    a.out_limit = plan.g.out_half;
-    a.combine = plan.combine;
-    a.out_mode = params->output;
-    a.dc = (params->flags & OSTFT_FLAG_REMOVE_DC) ? 1 : 0;
-    a.tw = nullptr;
-    a.tw_n = get_twiddles(ctx, 2 * M, M, &rc);  // W_N^k, k < M
-    if (!a.tw_n)
+    int log_m = 0;
+    const int rc = ostft::ctx_prepare(ctx, params, d_in, full_size, d_out, out_cap, &a, &log_m);
+    if (rc != OSTFT_OK)
         return rc;
-
+    hipStream_t s = static_cast<hipStream_t>(hip_stream);
     if (a.frames == 0)
         return OSTFT_OK;
-    if (ostft::fused_supported(plan.log_m)) {
-        a.tw = get_twiddles(ctx, M, M, &rc);
-        if (!a.tw)
-            return rc;
-        hipError_t e = ostft::launch_fused(plan.log_m, a, s);
+    if (ostft::fused_supported(log_m)) {
+        hipError_t e = ostft::launch_fused(log_m, a, s);
         if (e != hipSuccess)
             return hip_fail(ctx, e, "fused STFT launch");
         return OSTFT_OK;
     }
-    return ostft::run_multipass(ctx, plan.log_m, a, s);
+    return ostft::run_multipass(ctx, log_m, a, s);
+}
+
+int ostft_run_batch_device(ostft_ctx* ctx, const ostft_job* jobs, uint32_t njobs, void* hip_stream)
+{
+    if (!ctx || (!jobs && njobs > 0))
+        return fail(ctx, OSTFT_ERR_INVALID_ARG, "null argument");
+    DeviceGuard dg(ctx->device);
+    if (!dg.ok)
+        return fail(ctx, OSTFT_ERR_NO_DEVICE, "hipSetDevice failed");
+    return ostft::run_batch(ctx, jobs, njobs, static_cast<hipStream_t>(hip_stream));
+}
+
+int ostft_batch_config(ostft_ctx* ctx, int32_t exp, int32_t blocks_per_cu, int32_t use_graphs, int32_t* current)
+{
+    if (!ctx)
+        return OSTFT_ERR_INVALID_ARG;
+    DeviceGuard dg(ctx->device);
+    if (!dg.ok)
+        return fail(ctx, OSTFT_ERR_NO_DEVICE, "hipSetDevice failed");
+    return ostft::batch_config(ctx, exp, blocks_per_cu, use_graphs, current);
+}
+
+int ostft_autotune_batch(ostft_ctx* ctx, int32_t exp, uint64_t samples, int32_t* best_blocks_per_cu,
+                         double* best_ms)
+{
+    if (!ctx)
+        return OSTFT_ERR_INVALID_ARG;
+    DeviceGuard dg(ctx->device);
+    if (!dg.ok)
+        return fail(ctx, OSTFT_ERR_NO_DEVICE, "hipSetDevice failed");
+    return ostft::autotune_batch(ctx, exp, samples, best_blocks_per_cu, best_ms);
 }
 
 int ostft_activate(ostft_ctx* ctx, const float* in, uint64_t full_size, int32_t exp, float overlap,

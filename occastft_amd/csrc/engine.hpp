@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstddef>
 #include <cstdint>
 
 namespace ostft {
@@ -37,6 +38,32 @@ struct FusedArgs {
 // Single-kernel path: complex size M = 2^log_m, 5 <= log_m <= 13 (N = 64 .. 16384).
 bool fused_supported(int log_m);
 hipError_t launch_fused(int log_m, const FusedArgs& a, hipStream_t s);
+
+// ---- batches: many requests of one window size in one persistent launch ----
+// A batch descriptor block lives in device memory (uploaded by the host before
+// the launch); the kernel arguments only name where, so one captured launch
+// sequence serves every batch.
+struct BatchJob {
+    FusedArgs a;
+    uint64_t tile_begin;  // first tile of this request in its class
+};
+struct BatchClassHeader {
+    uint64_t jobs_off;    // byte offset of BatchJob[njobs] from the block base
+    uint64_t tiles_off;   // byte offset of uint32 tile_job[ntiles]
+    uint32_t ntiles;      // tiles of G frames in the class (0: the launch exits at once)
+    uint32_t counter;     // next unclaimed tile; uploaded as 0
+};
+struct BatchClassArgs {
+    void* base;           // descriptor block
+    uint64_t header_off;  // byte offset of this class's BatchClassHeader
+};
+struct BatchClassInfo {
+    int frames_per_tile;    // G
+    int threads;            // workgroup size
+    int max_blocks_per_cu;  // occupancy limit of the batch kernel
+};
+hipError_t launch_batch(int log_m, const BatchClassArgs& b, uint32_t grid, hipStream_t s);
+hipError_t batch_class_info(int log_m, BatchClassInfo* info);
 
 // FP64 table generators (run once per table).
 // tw[j] = exp(-2 pi i j / m) for j < count (count <= m)

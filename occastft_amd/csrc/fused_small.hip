@@ -37,24 +37,68 @@ namespace ostft {
 #ifndef OSTFT_PAIR_LDS
 #define OSTFT_PAIR_LDS 1
 #endif
+// Minimum waves per SIMD the persistent batch kernel is compiled for: 4 (a
+// 128-register budget, which LDS allows anyway) where that compiles without
+// spills, else 1 (the compiler's choice).
+template <int LOG_M>
+struct BatchWaves {
+    static constexpr int value = (LOG_M == 9 || LOG_M == 13) ? 1 : 4;
+};
+
+// FusedArgs as the batch kernel reads it from the descriptor block: the same
+// layout, with the pointers typed as global memory.  A generic pointer read
+// from memory would compile to flat accesses; one cast from a global pointer
+// lets the compiler use global accesses, as it does for kernel arguments.
+#define OSTFT_GLOBAL __attribute__((address_space(1)))
+struct FusedArgsGlobal {
+    const float OSTFT_GLOBAL* in;
+    uint64_t L, hop, frames;
+    const float OSTFT_GLOBAL* win;
+    const float2 OSTFT_GLOBAL* tw;
+    const float2 OSTFT_GLOBAL* tw_n;
+    float OSTFT_GLOBAL* out;
+    uint64_t out_limit;
+    int32_t combine, out_mode, dc;
+};
+static_assert(sizeof(FusedArgsGlobal) == sizeof(FusedArgs), "FusedArgsGlobal mirrors FusedArgs");
+static_assert(offsetof(FusedArgsGlobal, out) == offsetof(FusedArgs, out), "FusedArgsGlobal mirrors FusedArgs");
+static_assert(offsetof(FusedArgsGlobal, dc) == offsetof(FusedArgs, dc), "FusedArgsGlobal mirrors FusedArgs");
+
+__device__ __forceinline__ FusedArgs from_global(const FusedArgsGlobal& g)
+{
+    FusedArgs a;
+    a.in = (const float*)g.in;
+    a.L = g.L;
+    a.hop = g.hop;
+    a.frames = g.frames;
+    a.win = (const float*)g.win;
+    a.tw = (const float2*)g.tw;
+    a.tw_n = (const float2*)g.tw_n;
+    a.out = (float*)g.out;
+    a.out_limit = g.out_limit;
+    a.combine = g.combine;
+    a.out_mode = g.out_mode;
+    a.dc = g.dc;
+    return a;
+}
 
 // LDS float index of staged sample q: 4 pad floats after every 128, which
 // keeps 16-byte rows aligned for the vector writes.
 __device__ __forceinline__ uint32_t spad(uint32_t q) { return q + ((q >> 7) << 2); }
 
+// The work of one workgroup: frames f0 .. f0+G-1 of request `a` (fewer at the
+// end of the request).  Shared by the one-request kernel and the persistent
+// batch kernel, so a frame's result does not depend on which of them ran it.
 template <int LOG_M>
-__global__ void __launch_bounds__(Cfg<LOG_M>::WG) stft_fused_kernel(const FusedArgs a)
+__device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t f0, float2* lds, float2* red,
+                                             const uint32_t tid)
 {
     using C = Cfg<LOG_M>;
     constexpr int M = C::M, N = 2 * M, T = C::T, G = C::G, WG = C::WG, PADM = C::PADM;
-    __shared__ float2 lds[G * PADM];
-    __shared__ float2 red[WG / 64];
     float* sf = reinterpret_cast<float*>(lds);
 
-    const uint32_t tid = threadIdx.x;
     const uint32_t grp = tid / T, t = tid % T;
-    const uint64_t f0 = (uint64_t)xcd_block(blockIdx.x, gridDim.x) * G;  // first frame of the block
-    const uint32_t nf = (uint32_t)min((uint64_t)G, a.frames - f0);      // frames in this block
+    const uint32_t nf = (uint32_t)min((uint64_t)G, a.frames - f0);  // frames in this tile
     const bool active = grp < nf;
     const uint64_t L = a.L, hop = a.hop;
     const float* __restrict__ in = a.in;
@@ -254,6 +298,53 @@ __global__ void __launch_bounds__(Cfg<LOG_M>::WG) stft_fused_kernel(const FusedA
 }
 
 template <int LOG_M>
+__global__ void __launch_bounds__(Cfg<LOG_M>::WG) stft_fused_kernel(const FusedArgs a)
+{
+    using C = Cfg<LOG_M>;
+    __shared__ float2 lds[C::G * C::PADM];
+    __shared__ float2 red[C::WG / 64];
+    fused_frames<LOG_M>(a, (uint64_t)xcd_block(blockIdx.x, gridDim.x) * C::G, lds, red, threadIdx.x);
+}
+
+// Persistent form for a batch of requests of one window size: every
+// workgroup claims tiles (G frames of one request) from the class's counter
+// until the tiles run out.  tile_job[t] names the request of tile t; the
+// request's first tile is jobs[j].tile_begin.
+template <int LOG_M>
+__global__ void __launch_bounds__(Cfg<LOG_M>::WG) __attribute__((amdgpu_waves_per_eu(BatchWaves<LOG_M>::value)))
+    stft_batch_kernel(const char* __restrict__ blk, uint32_t* __restrict__ counter, const uint32_t header_off)
+{
+    // `blk` is read-only here (the counter is reached only through `counter`),
+    // which lets the compiler read the descriptors with scalar loads
+    using C = Cfg<LOG_M>;
+    __shared__ float2 lds[C::G * C::PADM];
+    __shared__ float2 red[C::WG / 64];
+    __shared__ uint32_t claim;
+    const BatchClassHeader* h = reinterpret_cast<const BatchClassHeader*>(blk + header_off);
+    const uint32_t ntiles = h->ntiles;
+    const BatchJob* jobs = reinterpret_cast<const BatchJob*>(blk + h->jobs_off);
+    const uint32_t* tile_job = reinterpret_cast<const uint32_t*>(blk + h->tiles_off);
+    for (;;) {
+        if (threadIdx.x == 0)
+            claim = atomicAdd(counter, 1u);
+        __syncthreads();
+        const uint32_t tile = __builtin_amdgcn_readfirstlane(claim);
+        if (tile >= ntiles)
+            break;  // uniform: every thread read the same claim
+        const uint32_t j = tile_job[tile];
+        const FusedArgs a = from_global(*reinterpret_cast<const FusedArgsGlobal*>(&jobs[j].a));
+        // the thread index, re-stated opaquely every tile: otherwise the
+        // compiler hoists every thread-dependent address out of the loop and
+        // keeps them all live in registers (about twice the one-request
+        // kernel's budget)
+        uint32_t tid = threadIdx.x;
+        asm volatile("" : "+v"(tid));
+        fused_frames<LOG_M>(a, (uint64_t)(tile - jobs[j].tile_begin) * C::G, lds, red, tid);
+        __syncthreads();  // LDS and `claim` are reused by the next tile
+    }
+}
+
+template <int LOG_M>
 static hipError_t launch_t(const FusedArgs& a, hipStream_t s)
 {
     using C = Cfg<LOG_M>;
@@ -266,7 +357,63 @@ static hipError_t launch_t(const FusedArgs& a, hipStream_t s)
     return hipGetLastError();
 }
 
+template <int LOG_M>
+static hipError_t launch_batch_t(const BatchClassArgs& b, uint32_t grid, hipStream_t s)
+{
+    if (grid == 0)
+        return hipErrorInvalidValue;
+    char* blk = static_cast<char*>(b.base);
+    uint32_t* counter = reinterpret_cast<uint32_t*>(blk + b.header_off + offsetof(BatchClassHeader, counter));
+    hipLaunchKernelGGL(stft_batch_kernel<LOG_M>, dim3(grid), dim3(Cfg<LOG_M>::WG), 0, s,
+                       static_cast<const char*>(blk), counter, (uint32_t)b.header_off);
+    return hipGetLastError();
+}
+
+template <int LOG_M>
+static hipError_t batch_info_t(BatchClassInfo* info)
+{
+    info->frames_per_tile = Cfg<LOG_M>::G;
+    info->threads = Cfg<LOG_M>::WG;
+    int per_cu = 0;
+    const hipError_t e =
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, stft_batch_kernel<LOG_M>, Cfg<LOG_M>::WG, 0);
+    info->max_blocks_per_cu = per_cu;
+    return e;
+}
+
 bool fused_supported(int log_m) { return log_m >= 5 && log_m <= 13; }
+
+hipError_t launch_batch(int log_m, const BatchClassArgs& b, uint32_t grid, hipStream_t s)
+{
+    switch (log_m) {
+    case 5: return launch_batch_t<5>(b, grid, s);
+    case 6: return launch_batch_t<6>(b, grid, s);
+    case 7: return launch_batch_t<7>(b, grid, s);
+    case 8: return launch_batch_t<8>(b, grid, s);
+    case 9: return launch_batch_t<9>(b, grid, s);
+    case 10: return launch_batch_t<10>(b, grid, s);
+    case 11: return launch_batch_t<11>(b, grid, s);
+    case 12: return launch_batch_t<12>(b, grid, s);
+    case 13: return launch_batch_t<13>(b, grid, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t batch_class_info(int log_m, BatchClassInfo* info)
+{
+    switch (log_m) {
+    case 5: return batch_info_t<5>(info);
+    case 6: return batch_info_t<6>(info);
+    case 7: return batch_info_t<7>(info);
+    case 8: return batch_info_t<8>(info);
+    case 9: return batch_info_t<9>(info);
+    case 10: return batch_info_t<10>(info);
+    case 11: return batch_info_t<11>(info);
+    case 12: return batch_info_t<12>(info);
+    case 13: return batch_info_t<13>(info);
+    default: return hipErrorInvalidValue;
+    }
+}
 
 hipError_t launch_fused(int log_m, const FusedArgs& a, hipStream_t s)
 {

@@ -20,6 +20,7 @@
 #include <unistd.h>
 
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <iostream>
 #include <mutex>
@@ -74,11 +75,14 @@ public:
     }
 
     // Runner::ActivateSTFT: returns false where the reference returns nullopt.
+    // Requests go to the continuous-batching server, so requests of
+    // concurrent connections share batches on the GPU; a request whose input
+    // and output do not fit in the server's arena runs on its own through a
+    // context of its own.
     bool ActivateSTFT(const std::vector<float>& in, int exp, float overlap, const std::string& options,
                       std::vector<float>& out, std::string& why)
     {
-        std::lock_guard<std::mutex> lk(engine_mu_);  // one context, calls must not overlap
-        if (!ctx_) {
+        if (!srv_) {
             why = "no HIP device";
             return false;
         }
@@ -91,14 +95,29 @@ public:
             why = ostft_strerror(rc);
             return false;
         }
-        out.resize(g.out_floats);
-        uint64_t n = 0;
-        rc = ostft_activate(ctx_, in.empty() ? nullptr : in.data(), in.size(), exp, overlap, options.c_str(),
-                            out.data(), out.size(), &n);
-        if (rc != OSTFT_OK) {
-            why = std::string(ostft_strerror(rc)) + ": " + ostft_last_error(ctx_);
-            return false;
+        out.resize(g.out_floats ? g.out_floats : 1);
+        uint64_t ticket = 0, n = 0;
+        rc = ostft_server_submit(srv_, in.empty() ? nullptr : in.data(), in.size(), &p, out.data(), out.size(),
+                                 &ticket);
+        if (rc == OSTFT_OK) {
+            rc = ostft_server_wait(srv_, ticket, -1, &n);
+            if (rc != OSTFT_OK)
+                why = std::string(ostft_strerror(rc)) + ": " + ostft_server_last_error(srv_);
+        } else if (rc == OSTFT_ERR_TOO_LARGE) {
+            std::lock_guard<std::mutex> lk(big_mu_);
+            if (!big_ctx_ && ostft_create(0, &big_ctx_) != OSTFT_OK)
+                big_ctx_ = nullptr;
+            rc = big_ctx_ ? ostft_activate(big_ctx_, in.empty() ? nullptr : in.data(), in.size(), exp, overlap,
+                                           options.c_str(), out.data(), out.size(), &n)
+                          : OSTFT_ERR_NO_DEVICE;
+            if (rc != OSTFT_OK)
+                why = std::string(ostft_strerror(rc)) + (big_ctx_ ? std::string(": ") + ostft_last_error(big_ctx_)
+                                                                  : std::string());
+        } else {
+            why = std::string(ostft_strerror(rc)) + ": " + ostft_server_last_error(srv_);
         }
+        if (rc != OSTFT_OK)
+            return false;
         out.resize(n);
         return true;
     }
@@ -106,17 +125,32 @@ public:
 private:
     void InitEnv()
     {
-        if (ostft_create(0, &ctx_) != OSTFT_OK) {
-            ctx_ = nullptr;
+        ostft_server_config cfg;
+        std::memset(&cfg, 0, sizeof(cfg));
+        cfg.use_graphs = 1;
+        if (const char* a = std::getenv("OSTFT_ARENA_MB"))
+            cfg.arena_bytes = (uint64_t)std::strtoull(a, nullptr, 10) << 20;
+        if (ostft_server_create(0, &cfg, &srv_) != OSTFT_OK) {
+            srv_ = nullptr;
             std::cerr << "hipRun: no HIP device; every request will be answered with ERR" << std::endl;
         }
     }
     void BuildKernel() {}  // kernels are compiled ahead of time into the engine library
     void UnInit()
     {
-        if (ctx_)
-            ostft_destroy(ctx_);
-        ctx_ = nullptr;
+        if (srv_) {
+            ostft_server_stats st;
+            if (ostft_server_stats_get(srv_, &st, 0) == OSTFT_OK && st.completed > 0)
+                std::fprintf(stderr,
+                             "hipRun: %llu requests in %llu batches, latency p50 %.3f ms p99 %.3f ms\n",
+                             (unsigned long long)st.completed, (unsigned long long)st.batches, st.latency_p50_ms,
+                             st.latency_p99_ms);
+            ostft_server_destroy(srv_);
+        }
+        srv_ = nullptr;
+        if (big_ctx_)
+            ostft_destroy(big_ctx_);
+        big_ctx_ = nullptr;
     }
 
     // RunnerCommon.cpp AccessData: the input from shm or from the data list.
@@ -212,8 +246,9 @@ private:
     }
 
     ostft_wire::WsServer server_;
-    ostft_ctx* ctx_ = nullptr;
-    std::mutex engine_mu_;
+    ostft_server* srv_ = nullptr;
+    ostft_ctx* big_ctx_ = nullptr;  // requests larger than the server's arena
+    std::mutex big_mu_;
 };
 
 }  // namespace

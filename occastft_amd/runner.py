@@ -8,6 +8,10 @@ behaviour on a pipeline error, StandAlone/cross_gpgpu/CUDA/src/CudaImpl.cpp:497-
 ``Runner.stft`` is the same call raising ``STFTError`` instead.
 ``Runner.run_device`` is the device-resident form over torch tensors: torch is
 used only for device memory and the stream; the work is the HIP engine.
+``Runner.run_batch_device`` runs many such requests in one launch sequence, and
+``Server`` is the continuous-batching engine for host-buffer requests (neither
+has a counterpart in the reference, which serves one request per
+ActivateSTFT call).
 """
 from __future__ import annotations
 
@@ -127,3 +131,124 @@ class Runner:
         if rc != N.OK:
             raise STFTError(rc, self.last_error)
         return out[: g["out_floats"]]
+
+    # -- batches -----------------------------------------------------------
+    def run_batch_device(self, jobs, stream=None):
+        """Run [(x, params, out), ...] of float32 CUDA tensors in one batch
+        (ostft_run_batch_device); `out` may be None (allocated).  Returns the
+        output tensors, each trimmed to its layout's length."""
+        import torch
+
+        arr = (N.Job * max(len(jobs), 1))()
+        outs = []
+        dev = None
+        for i, (x, params, out) in enumerate(jobs):
+            if x.dtype != torch.float32 or not x.is_cuda or not x.is_contiguous():
+                raise ValueError("inputs must be contiguous float32 tensors on the GPU")
+            dev = x.device
+            try:
+                n_out = geometry(params, x.numel())["out_floats"]
+            except STFTError:
+                n_out = 0  # the engine rejects the batch and names this job
+            if out is None:
+                out = torch.empty(max(n_out, 4), dtype=torch.float32, device=x.device)
+            arr[i].in_ptr = x.data_ptr()
+            arr[i].full_size = x.numel()
+            arr[i].out_ptr = out.data_ptr()
+            arr[i].out_cap = out.numel()
+            arr[i].params = params
+            outs.append(out[:n_out])
+        if not jobs:
+            return []
+        s = stream if stream is not None else torch.cuda.current_stream(dev)
+        rc = N.lib().ostft_run_batch_device(self._h, arr, len(jobs), ctypes.c_void_p(s.cuda_stream))
+        if rc != N.OK:
+            raise STFTError(rc, self.last_error)
+        return outs
+
+    def batch_config(self, exp: int = 0, blocks_per_cu: int = 0, use_graphs: int = -1) -> int:
+        cur = ctypes.c_int32()
+        rc = N.lib().ostft_batch_config(self._h, int(exp), int(blocks_per_cu), int(use_graphs), ctypes.byref(cur))
+        if rc != N.OK:
+            raise STFTError(rc, self.last_error)
+        return cur.value
+
+    def autotune_batch(self, exp: int, samples: int = 1 << 24):
+        """-> (best workgroups per CU, best ms) for window exponent `exp`."""
+        best = ctypes.c_int32()
+        ms = ctypes.c_double()
+        rc = N.lib().ostft_autotune_batch(self._h, int(exp), int(samples), ctypes.byref(best), ctypes.byref(ms))
+        if rc != N.OK:
+            raise STFTError(rc, self.last_error)
+        return best.value, ms.value
+
+
+class Server:
+    """Continuous-batching engine for host-buffer requests (ostft_server_*).
+
+    ``submit`` queues a request and returns a ticket; ``wait`` returns the
+    result array.  Input and output arrays are kept referenced here until the
+    request is waited for."""
+
+    def __init__(self, device: int = 0, arena_bytes: int = 0, max_batch: int = 0, max_inflight: int = 0,
+                 use_graphs: bool = True, autotune: bool = False):
+        cfg = N.ServerConfig(int(arena_bytes), int(max_batch), int(max_inflight), 1 if use_graphs else 0,
+                             1 if autotune else 0)
+        h = ctypes.c_void_p()
+        rc = N.lib().ostft_server_create(int(device), ctypes.byref(cfg), ctypes.byref(h))
+        if rc != N.OK:
+            raise STFTError(rc, f"server on device {device}")
+        self._h = h
+        self._live = {}
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            N.lib().ostft_server_destroy(self._h)
+            self._h = None
+            self._live.clear()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def last_error(self) -> str:
+        return N.lib().ostft_server_last_error(self._h).decode()
+
+    def submit(self, x: np.ndarray, params: N.Params, out: Optional[np.ndarray] = None) -> int:
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        g = geometry(params, x.size)
+        if out is None:
+            out = np.empty(max(g["out_floats"], 1), dtype=np.float32)
+        t = ctypes.c_uint64()
+        rc = N.lib().ostft_server_submit(self._h, ctypes.c_void_p(x.ctypes.data), x.size, ctypes.byref(params),
+                                         ctypes.c_void_p(out.ctypes.data), out.size, ctypes.byref(t))
+        if rc != N.OK:
+            raise STFTError(rc, self.last_error)
+        self._live[t.value] = (x, out)
+        return t.value
+
+    def wait(self, ticket: int, timeout_ms: int = -1) -> np.ndarray:
+        n = ctypes.c_uint64()
+        rc = N.lib().ostft_server_wait(self._h, int(ticket), int(timeout_ms), ctypes.byref(n))
+        if rc == N.ERR_TIMEOUT:
+            raise TimeoutError(f"request {ticket} still running")
+        x, out = self._live.pop(ticket, (None, None))
+        if rc != N.OK:
+            raise STFTError(rc, self.last_error)
+        return out[: n.value]
+
+    def stats(self, reset: bool = False) -> dict:
+        s = N.ServerStats()
+        rc = N.lib().ostft_server_stats_get(self._h, ctypes.byref(s), 1 if reset else 0)
+        if rc != N.OK:
+            raise STFTError(rc)
+        return s.as_dict()
