@@ -20,6 +20,21 @@ measured for that kernel (profiles/traffic.json), when profiled.
 cpu_baseline = the reference's own OpenMP kernels (oracle/_ref, compiled from
 the reference sources) or, without them, the oracle's OpenMP port, timed on the
 host cores over a bounded prefix of the same signal and the same 28 requests.
+
+Other workloads (--workload; one JSON line each, same fields):
+  large  -- 2^27-sample signal (N(0,1) noise + 0.5-amplitude linear chirp
+            0 -> 0.5 cycles/sample), windows 2^16..2^20, overlap 0 and 75%,
+            Blackman + DC removal, power spectrum (N/2+1 bins): the multi-pass
+            path; 10 requests per step.
+  batch  -- 4096 signals of 2^18 samples of unit-variance 1/f (pink) noise,
+            windows 64, 256, 1024, overlap 50%, Hanning: one persistent batch
+            launch per window size (4096 requests each); 3 launches per step.
+  serve  -- 2000 requests arriving as a Poisson process (--rate per second),
+            lengths log-uniform in [2^14, 2^22] (U[-1,1] noise + one sinusoid),
+            window uniform over 2^8..2^13, overlap over {0, 25, 50, 75}%, window
+            type over Hanning/Hamming/Blackman, power spectrum: host buffers
+            through the continuous-batching server; p50/p99 latency from submit
+            to result in the caller's buffer.
 """
 from __future__ import annotations
 
@@ -165,6 +180,399 @@ def copy_probe(args) -> int:
     return 0
 
 
+def header(args, value, unit, ms_per_step, config, dtype="f32", higher_is_better=True):
+    return {
+        "metric": "frames/s & HBM GB/s (% of peak), plain vs fused/MFMA at fixed error; p99 ms",
+        "value": round(value, 1),
+        "unit": unit,
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": higher_is_better,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": dtype,
+        "data": "synthetic",
+        "config": config,
+    }
+
+
+def roofline(achieved_gbps, kernel, bytes_per_launch, launch_ms, traffic=None, **extra):
+    r = {"bound": "hbm", "achieved": round(achieved_gbps, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+         "frac": round(achieved_gbps / HBM_PEAK_GBPS, 4), "traffic": traffic, "kernel": kernel,
+         "bytes_per_launch": int(bytes_per_launch), "launch_ms": round(launch_ms, 4)}
+    r.update(extra)
+    return r
+
+
+def cold_time(fn, stream, reps):
+    """Mean ms of fn() between HIP events on `stream`, Infinity Cache evicted
+    (a 1 GiB buffer overwritten) before each call."""
+    import torch
+
+    flush = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
+    ts = []
+    for _ in range(reps):
+        flush.zero_()
+        s0, s1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s0.record(stream)
+        fn()
+        s1.record(stream)
+        torch.cuda.synchronize()
+        ts.append(s0.elapsed_time(s1))
+    del flush
+    return float(np.mean(ts))
+
+
+# ---------------------------------------------------------------- large windows
+
+LARGE_LEN = 1 << 27
+LARGE_EXPS = list(range(16, 21))
+LARGE_OVERLAPS = (0.0, 0.75)
+LARGE_OPTS = "--blackman_window --remove_dc --power_return"
+
+
+def make_chirp(L: int, seed: int):
+    """N(0,1) noise + 0.5 cos(phi(n)), phi = 2 pi (0.5 n^2 / (2L)): a linear
+    chirp from 0 to 0.5 cycles/sample; the phase is reduced exactly in int64
+    (n^2 mod 4L)."""
+    import torch
+
+    x = torch.empty(L, dtype=torch.float32, device="cuda")
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    chunk = 1 << 24
+    for s in range(0, L, chunk):
+        n = torch.arange(s, min(L, s + chunk), dtype=torch.int64, device="cuda")
+        ph = ((n * n) % (4 * L)).double() * (np.pi / (2 * L))
+        acc = torch.randn(n.shape, dtype=torch.float64, device="cuda", generator=g) + 0.5 * torch.cos(ph)
+        x[s: s + n.numel()] = acc.float()
+    return x
+
+
+def bench_large(args) -> dict:
+    import torch
+
+    import occastft_amd as ostft
+
+    runner = ostft.Runner(0)
+    x = make_chirp(LARGE_LEN, SEED + 3)
+    reqs = []
+    max_out = 0
+    for e in LARGE_EXPS:
+        for ov in LARGE_OVERLAPS:
+            p = ostft.parse_options(LARGE_OPTS, e, ov)
+            g = ostft.geometry(p, LARGE_LEN)
+            reqs.append((e, ov, p, g))
+            max_out = max(max_out, g["out_floats"])
+    out = torch.empty(max_out, dtype=torch.float32, device="cuda")
+    stream = torch.cuda.current_stream()
+    for _ in range(args.warmup):
+        for e, ov, p, g in reqs:
+            runner.run_device(x, p, out=out, stream=stream)
+    torch.cuda.synchronize()
+    ev = [[(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in reqs]
+          for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for st in range(args.steps):
+        for i, (e, ov, p, g) in enumerate(reqs):
+            ev[st][i][0].record(stream)
+            runner.run_device(x, p, out=out, stream=stream)
+            ev[st][i][1].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    per_ms = np.array([[ev[st][i][0].elapsed_time(ev[st][i][1]) for i in range(len(reqs))]
+                       for st in range(args.steps)]).mean(axis=0)
+    frames = sum(g["frames"] for *_, g in reqs)
+    ms_per_step = wall * 1000.0 / args.steps
+    rows = []
+    for (e, ov, p, g), ms in zip(reqs, per_ms):
+        b = algorithmic_bytes(g, 1 << e)
+        rows.append(dict(window=1 << e, overlap=ov, ms=round(float(ms), 4),
+                         frames_per_s=round(g["frames"] / (ms / 1e3)), gbps=round(b / (ms / 1e3) / 1e9, 1)))
+    dom = int(np.argmax(per_ms))
+    e, ov, p, g = reqs[dom]
+    b = algorithmic_bytes(g, 1 << e)
+    cold_ms = cold_time(lambda: runner.run_device(x, p, out=out, stream=stream), stream, max(3, args.steps))
+    total_bytes = sum(algorithmic_bytes(g, 1 << e) for e, ov, p, g in reqs)
+    res = header(args, frames / (ms_per_step / 1e3), "frames/s", ms_per_step, {
+        "workload": "plain FP32 large-window multi-pass path: 2^27-sample signal (N(0,1) + 0.5 linear chirp "
+                    "0->0.5 cycles/sample), windows 2^16..2^20, overlap 0/75%, Blackman + DC removal, power "
+                    "spectrum N/2+1 bins (10 requests per step)",
+        "signal_samples": LARGE_LEN, "requests_per_step": len(reqs), "frames_per_step": frames})
+    res["hbm_gbps_sweep"] = round(total_bytes / (ms_per_step / 1e3) / 1e9, 1)
+    res["roofline"] = roofline(b / (cold_ms / 1e3) / 1e9,
+                               f"multi-pass request window {1 << e} overlap {ov} (column passes + split, "
+                               f"algorithmic bytes of the request over its launches)", b, cold_ms,
+                               launch_ms_in_sweep=round(float(per_ms[dom]), 4))
+    res["sweep"] = rows
+    res["cpu_baseline"] = None
+    runner.close()
+    return res
+
+
+# ---------------------------------------------------------------- many signals
+
+BATCH_SIGNALS = 4096
+BATCH_LEN = 1 << 18
+BATCH_EXPS = (6, 8, 10)
+
+
+def make_pink(S: int, L: int, seed: int):
+    """S signals of unit-variance 1/f noise: white Gaussian noise shaped by
+    1/sqrt(f) in the frequency domain (DC removed), per-signal normalised."""
+    import torch
+
+    x = torch.empty(S, L, dtype=torch.float32, device="cuda")
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    f = torch.arange(L // 2 + 1, dtype=torch.float32, device="cuda")
+    shape = torch.where(f > 0, 1.0 / torch.sqrt(torch.clamp(f, min=1.0)), torch.zeros_like(f))
+    for s in range(0, S, 256):
+        w = torch.randn(min(256, S - s), L, dtype=torch.float32, device="cuda", generator=g)
+        y = torch.fft.irfft(torch.fft.rfft(w, dim=1) * shape, n=L, dim=1)
+        x[s: s + w.shape[0]] = y / y.std(dim=1, keepdim=True)
+    return x
+
+
+def batch_jobs(x, exp, out, opts="--hanning_window"):
+    """ctypes job array: every row of x with the same parameters, outputs packed
+    into `out` (16-byte aligned slices)."""
+    import occastft_amd as ostft
+    from occastft_amd import _native as N
+
+    S, L = x.shape
+    p = ostft.parse_options(opts, exp, 0.5)
+    g = ostft.geometry(p, L)
+    stride = (g["out_floats"] + 3) // 4 * 4
+    assert out.numel() >= S * stride
+    arr = (N.Job * S)()
+    for i in range(S):
+        arr[i].in_ptr = x[i].data_ptr()
+        arr[i].full_size = L
+        arr[i].out_ptr = out.data_ptr() + 4 * i * stride
+        arr[i].out_cap = g["out_floats"]
+        arr[i].params = p
+    return arr, p, g, stride
+
+
+def bench_batch(args) -> dict:
+    import torch
+
+    import occastft_amd as ostft
+    from occastft_amd import _native as N
+
+    runner = ostft.Runner(0)
+    x = make_pink(BATCH_SIGNALS, BATCH_LEN, SEED + 4)
+    plans = []
+    need = 0
+    for e in BATCH_EXPS:
+        p = ostft.parse_options("--hanning_window", e, 0.5)
+        g = ostft.geometry(p, BATCH_LEN)
+        need = max(need, BATCH_SIGNALS * ((g["out_floats"] + 3) // 4 * 4))
+    out = torch.empty(need, dtype=torch.float32, device="cuda")
+    for e in BATCH_EXPS:
+        arr, p, g, stride = batch_jobs(x, e, out)
+        plans.append((e, arr, p, g))
+    stream = torch.cuda.current_stream()
+    lib = N.lib()
+    sp = ctypes.c_void_p(stream.cuda_stream)
+
+    def launch(arr):
+        rc = lib.ostft_run_batch_device(runner._h, arr, BATCH_SIGNALS, sp)
+        if rc != N.OK:
+            raise ostft.STFTError(rc, runner.last_error)
+
+    for e, *_ in plans:
+        runner.autotune_batch(e, 1 << 27)
+    for _ in range(args.warmup):
+        for e, arr, p, g in plans:
+            launch(arr)
+    torch.cuda.synchronize()
+    ev = [[(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in plans]
+          for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for st in range(args.steps):
+        for i, (e, arr, p, g) in enumerate(plans):
+            ev[st][i][0].record(stream)
+            launch(arr)
+            ev[st][i][1].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    per_ms = np.array([[ev[st][i][0].elapsed_time(ev[st][i][1]) for i in range(len(plans))]
+                       for st in range(args.steps)]).mean(axis=0)
+    frames = sum(BATCH_SIGNALS * g["frames"] for *_, g in plans)
+    ms_per_step = wall * 1000.0 / args.steps
+    rows = []
+    for (e, arr, p, g), ms in zip(plans, per_ms):
+        b = BATCH_SIGNALS * algorithmic_bytes(g, 1 << e)
+        rows.append(dict(window=1 << e, requests=BATCH_SIGNALS, ms=round(float(ms), 4),
+                         frames_per_s=round(BATCH_SIGNALS * g["frames"] / (ms / 1e3)),
+                         gbps=round(b / (ms / 1e3) / 1e9, 1),
+                         blocks_per_cu=runner.batch_config(e)))
+    dom = int(np.argmax(per_ms))
+    e, arr, p, g = plans[dom]
+    b = BATCH_SIGNALS * algorithmic_bytes(g, 1 << e)
+    cold_ms = cold_time(lambda: launch(arr), stream, max(3, args.steps))
+    total_bytes = sum(BATCH_SIGNALS * algorithmic_bytes(g, 1 << e) for e, arr, p, g in plans)
+    res = header(args, frames / (ms_per_step / 1e3), "frames/s", ms_per_step, {
+        "workload": "many signals, plain FP32: 4096 signals x 2^18 samples of unit-variance pink noise, "
+                    "windows 64/256/1024, overlap 50%, Hanning, default power layout; one persistent batch "
+                    "launch of 4096 requests per window (3 per step)",
+        "signals": BATCH_SIGNALS, "samples_per_signal": BATCH_LEN, "frames_per_step": frames})
+    res["hbm_gbps_sweep"] = round(total_bytes / (ms_per_step / 1e3) / 1e9, 1)
+    res["roofline"] = roofline(b / (cold_ms / 1e3) / 1e9,
+                               f"stft_batch_kernel<{e - 1}> window {1 << e}, 4096 requests (one launch + "
+                               f"descriptor upload)", b, cold_ms, launch_ms_in_sweep=round(float(per_ms[dom]), 4))
+    res["sweep"] = rows
+    res["cpu_baseline"] = None
+    runner.close()
+    return res
+
+
+# ---------------------------------------------------------------- serving
+
+SERVE_REQUESTS = 2000
+SERVE_EXPS = tuple(range(8, 14))
+SERVE_OVERLAPS = (0.0, 0.25, 0.5, 0.75)
+SERVE_WINDOWS = ("--hanning_window", "--hamming_window", "--blackman_window")
+
+
+def make_trace(n: int, rate: float, seed: int):
+    """n requests: (arrival s, signal, params); arrivals Poisson at `rate`/s
+    (all at 0 when rate <= 0).  Signals are generated on the GPU and kept in
+    host memory, as a server receives them."""
+    import torch
+
+    import occastft_amd as ostft
+
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0 / rate, n) if rate > 0 else np.zeros(n)
+    arrivals = np.cumsum(gaps) - gaps[0]
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    reqs = []
+    for i in range(n):
+        L = int(np.exp(rng.uniform(np.log(2.0 ** 14), np.log(2.0 ** 22))))
+        e = int(rng.choice(SERVE_EXPS))
+        ov = float(rng.choice(SERVE_OVERLAPS))
+        opts = str(rng.choice(SERVE_WINDOWS)) + " --power_return"
+        f, a, ph = rng.uniform(0, 0.5), rng.uniform(0.1, 1.0), rng.uniform(0, 2 * np.pi)
+        t = torch.arange(L, dtype=torch.float64, device="cuda")
+        sig = (torch.rand(L, dtype=torch.float64, device="cuda", generator=g) * 2 - 1
+               + a * torch.sin(2 * np.pi * f * t + ph)).float()
+        reqs.append((float(arrivals[i]), sig.cpu().numpy(), ostft.parse_options(opts, e, ov)))
+    return reqs
+
+
+def replay(srv, reqs, outs):
+    """Submit on schedule from one thread, wait in order on this one; returns
+    the server's statistics for the replay."""
+    import queue
+    import threading
+
+    q = queue.Queue()
+
+    def submitter():
+        t0 = time.perf_counter()
+        for (t, x, p), out in zip(reqs, outs):
+            dt = t0 + t - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            q.put(srv.submit(x, p, out=out))
+        q.put(None)
+
+    th = threading.Thread(target=submitter)
+    th.start()
+    while True:
+        tk = q.get()
+        if tk is None:
+            break
+        srv.wait(tk)
+    th.join()
+    return srv.stats(reset=True)
+
+
+def bench_serve(args) -> dict:
+    import torch
+
+    import occastft_amd as ostft
+    from occastft_amd import _native as N
+
+    reqs = make_trace(SERVE_REQUESTS, args.rate, SEED + 5)
+    outs = []
+    for _, x, p in reqs:
+        o = np.empty(ostft.geometry(p, x.size)["out_floats"], np.float32)
+        o.fill(0.0)  # touch the pages: the server's copy-out should not pay first-touch faults
+        outs.append(o)
+    total_frames = sum(ostft.geometry(p, x.size)["frames"] for _, x, p in reqs)
+    srv = ostft.Server(0, arena_bytes=2 << 30, autotune=True)
+    # warm-up: every window size once (tables, captured launches)
+    warm = [(0.0, np.ones(1 << 16, np.float32), ostft.parse_options(o + " --power_return", e, 0.5))
+            for e in SERVE_EXPS for o in SERVE_WINDOWS]
+    wouts = [np.empty(ostft.geometry(p, x.size)["out_floats"], np.float32) for _, x, p in warm]
+    for _ in range(max(1, args.warmup)):
+        replay(srv, warm, wouts)
+    steps = max(1, min(args.steps, 3))
+    stats = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        stats.append(replay(srv, reqs, outs))
+    wall = time.perf_counter() - t0
+    srv.close()
+    st = stats[-1]
+    span = st["span_s"]
+    # the same requests resident in HBM, one batch: the persistent kernels'
+    # bandwidth on the ragged mix, measured cold
+    runner = ostft.Runner(0)
+    dins = [torch.from_numpy(x).cuda() for _, x, _ in reqs]
+    sizes = [(ostft.geometry(p, x.size)["out_floats"] + 3) // 4 * 4 for _, x, p in reqs]
+    dout = torch.empty(sum(sizes), dtype=torch.float32, device="cuda")
+    arr = (N.Job * len(reqs))()
+    off = 0
+    alg = 0
+    for i, ((_, x, p), xd, sz) in enumerate(zip(reqs, dins, sizes)):
+        g = ostft.geometry(p, x.size)
+        arr[i].in_ptr = xd.data_ptr()
+        arr[i].full_size = x.size
+        arr[i].out_ptr = dout.data_ptr() + 4 * off
+        arr[i].out_cap = g["out_floats"]
+        arr[i].params = p
+        off += sz
+        alg += algorithmic_bytes(g, 1 << p.exp)
+    for e in SERVE_EXPS:
+        runner.autotune_batch(e, 1 << 27)
+    stream = torch.cuda.current_stream()
+    sp = ctypes.c_void_p(stream.cuda_stream)
+    lib = N.lib()
+
+    def launch():
+        rc = lib.ostft_run_batch_device(runner._h, arr, len(reqs), sp)
+        if rc != N.OK:
+            raise ostft.STFTError(rc, runner.last_error)
+
+    launch()
+    torch.cuda.synchronize()
+    cold_ms = cold_time(launch, stream, 5)
+    runner.close()
+    res = header(args, st["frames"] / span if span > 0 else 0.0, "frames/s", wall * 1000.0 / steps, {
+        "workload": "ragged serving: 2000 requests, Poisson arrivals, lengths log-uniform [2^14, 2^22] "
+                    "(U[-1,1] + one sinusoid), windows 2^8..2^13, overlap 0/25/50/75%, Hanning/Hamming/"
+                    "Blackman, power spectrum N/2+1 bins; host buffers through the continuous-batching server "
+                    "(one step = one replay of the trace)",
+        "requests": SERVE_REQUESTS, "arrival_rate_per_s": args.rate, "frames_per_step": total_frames,
+        "arena_bytes": 2 << 30})
+    res["latency_ms"] = {"p50": round(st["latency_p50_ms"], 3), "p99": round(st["latency_p99_ms"], 3),
+                         "max": round(st["latency_max_ms"], 3), "mean": round(st["latency_mean_ms"], 3)}
+    res["serving"] = {"completed": st["completed"], "failed": st["failed"], "batches": st["batches"],
+                      "mean_batch": round(st["mean_batch"], 2), "span_s": round(span, 3),
+                      "requests_per_s": round(st["completed"] / span, 1) if span > 0 else None,
+                      "host_gbps": round((st["bytes_in"] + st["bytes_out"]) / span / 1e9, 2) if span else None}
+    res["roofline"] = roofline(alg / (cold_ms / 1e3) / 1e9,
+                               "stft_batch_kernel, the 2000 requests resident in HBM as one batch (6 window "
+                               "sizes: 6 persistent launches + descriptor upload)", alg, cold_ms)
+    res["cpu_baseline"] = None
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -174,6 +582,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only time a plain 1 GiB device copy (achievable HBM bandwidth; counter calibration)")
+    ap.add_argument("--workload", choices=("sweep", "large", "batch", "serve"), default="sweep")
+    ap.add_argument("--rate", type=float, default=1000.0, help="serve: Poisson arrival rate per second")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: frames are independent, so N GPUs would each run this "
@@ -187,6 +597,10 @@ def main() -> int:
     torch.cuda.set_device(0)
     if args.copy_probe:
         return copy_probe(args)
+    if args.workload != "sweep":
+        fn = {"large": bench_large, "batch": bench_batch, "serve": bench_serve}[args.workload]
+        print(json.dumps(fn(args)))
+        return 0
     runner = ostft.Runner(0)
     x = make_signal(SIGNAL_LEN, SEED)
     reqs = []

@@ -153,25 +153,29 @@ typedef struct ostft_job {
 } ostft_job;
 
 /* Runs `njobs` device-resident requests of any window size, overlap, window
- * and output layout: one persistent launch per window size present (windows
- * 2^6..2^14), whose workgroups pull tiles of frames from a device-side
- * counter; larger windows run after them, request by request.  Every
+ * and output layout: one launch per window size present (windows 2^6..2^14)
+ * over a device-side queue of tiles of frames, either one workgroup per tile
+ * or a persistent grid whose workgroups pull tiles from a counter; larger
+ * windows run after them, request by request.  Every
  * request's result is bitwise identical to running it alone with
  * ostft_run_device.  All requests are validated before anything is launched;
  * a bad one fails the call (its index is in ostft_last_error).  Queued on
  * `hip_stream`, no wait.  Outputs must not overlap. */
 int ostft_run_batch_device(ostft_ctx* ctx, const ostft_job* jobs, uint32_t njobs, void* hip_stream);
 
-/* Batch plan of window exponent `exp` (6..14): blocks_per_cu > 0 sets the
- * persistent workgroups per CU (clamped to the occupancy limit); use_graphs
- * 0/1 turns replay of captured launch sequences off/on (-1 leaves it; exp 0
- * sets only that); *current receives the setting in force. */
+/* Batch plan of window exponent `exp` (6..14): blocks_per_cu > 0 selects the
+ * persistent form with that many workgroups per CU (clamped to the occupancy
+ * limit), < 0 the one-workgroup-per-tile form (the default), 0 leaves it;
+ * use_graphs 0/1 turns replay of captured launch sequences off/on (-1 leaves
+ * it; exp 0 sets only that); *current receives the setting in force (0 for
+ * one workgroup per tile). */
 int ostft_batch_config(ostft_ctx* ctx, int32_t exp, int32_t blocks_per_cu, int32_t use_graphs,
                        int32_t* current);
 
-/* Times the persistent launch of window exponent `exp` on a synthetic ragged
- * batch of `samples` samples for every workgroups-per-CU setting, keeps the
- * fastest for this context and reports it. */
+/* Times window exponent `exp` on a synthetic ragged batch of `samples`
+ * samples in both forms (one workgroup per tile; persistent at every
+ * workgroups-per-CU setting), keeps the fastest for this context and reports
+ * it (*best_blocks_per_cu = 0 for one workgroup per tile). */
 int ostft_autotune_batch(ostft_ctx* ctx, int32_t exp, uint64_t samples, int32_t* best_blocks_per_cu,
                          double* best_ms);
 

@@ -35,6 +35,10 @@ def common_flags(defines: tuple = ()) -> list[str]:
         "-fPIC",
         # keep the compiler from forming packed-FP32 (v_pk_*_f32) instructions
         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+        # a one-lane atomic stays a plain atomic: the wave-scan rewrite waits
+        # for its result at once, which would stall the persistent batch
+        # kernel's prefetched tile claim behind the previous tile's stores
+        "-mllvm", "-amdgpu-atomic-optimizer-strategy=None",
         "-Wall",
         "-Wno-unused-function",
         "-I", os.path.join(HERE, "..", "include"),

@@ -1,6 +1,8 @@
 // batch.hip -- many requests of uneven length, window size and overlap in one
-// launch sequence: one persistent launch per window-size class present, each
-// pulling tiles (G frames of one request) from a device-side counter.
+// launch sequence: one launch per window-size class present over that class's
+// tile queue (tiles of G frames of one request), either one workgroup per tile
+// or a persistent grid pulling tiles from a device-side counter (measured per
+// class by the autotuner).
 //
 // The descriptor block (per-class headers, the requests' FusedArgs, the
 // tile -> request table) is built on the host in pinned memory and uploaded
@@ -48,17 +50,30 @@ struct BatchState {
     int next = 0;
     int cu_count = 0;
     int use_graphs = 1;
-    int per_cu[NCLS] = {};  // persistent workgroups per CU; 0: the occupancy limit
+    int persistent[NCLS] = {};  // 1: persistent grid claiming tiles; 0: one workgroup per tile
+    int per_cu[NCLS] = {};      // persistent workgroups per CU; 0: the occupancy limit
     BatchClassInfo info[NCLS] = {};
     bool info_ok[NCLS] = {};
-    std::map<std::pair<uint32_t, int>, hipGraphExec_t> graphs;  // (class mask, slot) -> replayable launches
+    // per slot: the launches of every class, captured once; a batch sets
+    // each class node's grid (its persistent grid, or one workgroup when the
+    // class has no tiles) and replays it
+    hipGraph_t graph[2] = {};
+    hipGraphExec_t exec[2] = {};
+    hipGraphNode_t node[2][NCLS] = {};
+    uint32_t node_grid[2][NCLS] = {};
+    int node_persistent[2][NCLS] = {};
     hipStream_t capture = nullptr;
 
     void drop_graphs()
     {
-        for (auto& kv : graphs)
-            (void)hipGraphExecDestroy(kv.second);
-        graphs.clear();
+        for (int i = 0; i < 2; ++i) {
+            if (exec[i])
+                (void)hipGraphExecDestroy(exec[i]);
+            if (graph[i])
+                (void)hipGraphDestroy(graph[i]);
+            exec[i] = nullptr;
+            graph[i] = nullptr;
+        }
     }
 };
 
@@ -104,14 +119,18 @@ int class_info(ostft_ctx* ctx, BatchState* st, int c)
     return OSTFT_OK;
 }
 
-uint32_t class_grid(const BatchState* st, int c)
+// Grid of class c for a batch with `ntiles` tiles of it.
+uint32_t class_grid(const BatchState* st, int c, uint64_t ntiles)
 {
+    if (!st->persistent[c])
+        return (uint32_t)std::max<uint64_t>(ntiles, 1);
     const int mx = st->info[c].max_blocks_per_cu;
     const int pc = st->per_cu[c] > 0 ? std::min(st->per_cu[c], mx) : mx;
     return (uint32_t)pc * (uint32_t)st->cu_count;
 }
 
-hipError_t launch_classes(BatchState* st, uint32_t mask, void* base, hipStream_t s)
+hipError_t launch_classes(BatchState* st, uint32_t mask, void* base, hipStream_t s, const uint64_t* ntiles,
+                          bool one_block = false)
 {
     for (int c = 0; c < NCLS; ++c) {
         if (!(mask >> c & 1u))
@@ -119,43 +138,100 @@ hipError_t launch_classes(BatchState* st, uint32_t mask, void* base, hipStream_t
         BatchClassArgs a;
         a.base = base;
         a.header_off = (uint64_t)c * sizeof(BatchClassHeader);
-        const hipError_t e = launch_batch(c + LOG_M_MIN, a, class_grid(st, c), s);
+        const uint32_t grid = one_block ? 1u : class_grid(st, c, ntiles[c]);
+        const hipError_t e = launch_batch(c + LOG_M_MIN, a, grid, st->persistent[c] != 0, s);
         if (e != hipSuccess)
             return e;
     }
     return hipSuccess;
 }
 
-// The launches for `mask` reading slot `si`, captured once.
-hipError_t graph_for(BatchState* st, uint32_t mask, int si, hipGraphExec_t* out)
+// Captures the launches of every class for slot `si` (grids of one
+// workgroup; set per batch) and finds each class's kernel node.
+hipError_t capture_slot(BatchState* st, int si)
 {
-    const auto key = std::make_pair(mask, si);
-    auto it = st->graphs.find(key);
-    if (it != st->graphs.end()) {
-        *out = it->second;
-        return hipSuccess;
-    }
+    const uint32_t all = (1u << NCLS) - 1u;
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamBeginCapture(st->capture, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess)
         return e;
-    const hipError_t le = launch_classes(st, mask, st->slot[si].d, st->capture);
+    const hipError_t le = launch_classes(st, all, st->slot[si].d, st->capture, nullptr, true);
     e = hipStreamEndCapture(st->capture, &g);
-    if (le != hipSuccess) {
+    if (le != hipSuccess || e != hipSuccess) {
         if (g)
             (void)hipGraphDestroy(g);
-        return le;
+        return le != hipSuccess ? le : e;
     }
-    if (e != hipSuccess)
-        return e;
+    // the captured graph is the chain class 0 -> class NCLS-1
+    for (int c = 0; c < NCLS; ++c)
+        st->node[si][c] = nullptr;
+    size_t n = 0;
+    e = hipGraphGetNodes(g, nullptr, &n);
+    std::vector<hipGraphNode_t> nodes(n);
+    if (e == hipSuccess && n == (size_t)NCLS)
+        e = hipGraphGetNodes(g, nodes.data(), &n);
+    else if (e == hipSuccess)
+        e = hipErrorInvalidValue;
+    for (size_t i = 0; e == hipSuccess && i < n; ++i) {
+        hipKernelNodeParams kp;
+        e = hipGraphKernelNodeGetParams(nodes[i], &kp);
+        if (e != hipSuccess)
+            break;
+        int c = -1;
+        for (int k = 0; k < NCLS; ++k)
+            if (kp.func == batch_kernel_fn(k + LOG_M_MIN, st->persistent[k] != 0))
+                c = k;
+        if (c < 0 || st->node[si][c]) {
+            e = hipErrorInvalidValue;
+            break;
+        }
+        st->node[si][c] = nodes[i];
+        st->node_grid[si][c] = 1;
+        st->node_persistent[si][c] = st->persistent[c];
+    }
     hipGraphExec_t ex = nullptr;
-    e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    if (e != hipSuccess)
+    if (e == hipSuccess)
+        e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(g);
         return e;
-    st->graphs[key] = ex;
-    *out = ex;
+    }
+    st->graph[si] = g;
+    st->exec[si] = ex;
     return hipSuccess;
+}
+
+// Replays slot `si`'s launches with the grids this batch needs.
+hipError_t replay_slot(BatchState* st, int si, uint32_t mask, const uint64_t* ntiles, hipStream_t s)
+{
+    for (int c = 0; st->exec[si] && c < NCLS; ++c)
+        if (st->node_persistent[si][c] != st->persistent[c]) {
+            // a class changed form: its node names another kernel
+            (void)hipGraphExecDestroy(st->exec[si]);
+            (void)hipGraphDestroy(st->graph[si]);
+            st->exec[si] = nullptr;
+            st->graph[si] = nullptr;
+        }
+    if (!st->exec[si]) {
+        const hipError_t e = capture_slot(st, si);
+        if (e != hipSuccess)
+            return e;
+    }
+    for (int c = 0; c < NCLS; ++c) {
+        const uint32_t want = (mask >> c & 1u) ? class_grid(st, c, ntiles[c]) : 1u;
+        if (want == st->node_grid[si][c])
+            continue;
+        hipKernelNodeParams kp;
+        hipError_t e = hipGraphKernelNodeGetParams(st->node[si][c], &kp);
+        if (e != hipSuccess)
+            return e;
+        kp.gridDim = dim3(want);
+        e = hipGraphExecKernelNodeSetParams(st->exec[si], st->node[si][c], &kp);
+        if (e != hipSuccess)
+            return e;
+        st->node_grid[si][c] = want;
+    }
+    return hipGraphLaunch(st->exec[si], s);
 }
 
 hipError_t ensure_slot(BatchState* st, int si, size_t bytes)
@@ -228,8 +304,8 @@ int run_batch(ostft_ctx* ctx, const ostft_job* jobs, uint32_t njobs, hipStream_t
         mask |= 1u << c;
     }
     for (int c = 0; c < NCLS; ++c)
-        if (ntiles[c] + class_grid(st, c) >= 0xffffffffull)
-            return ctx_fail(ctx, OSTFT_ERR_TOO_LARGE, "more than 2^32 tiles of one window size in a batch");
+        if (ntiles[c] >= 0x7fffffffull - 2ull * (uint64_t)st->cu_count * 64u)  // grid limit; claims overrun by <= 2/workgroup
+            return ctx_fail(ctx, OSTFT_ERR_TOO_LARGE, "more than 2^31 tiles of one window size in a batch");
     if (mask) {
         // 3. descriptor block layout
         size_t off = align_up(sizeof(BatchClassHeader) * NCLS);
@@ -281,12 +357,9 @@ int run_batch(ostft_ctx* ctx, const ostft_job* jobs, uint32_t njobs, hipStream_t
         if (e != hipSuccess)
             return ctx_hip_fail(ctx, e, "batch descriptor upload");
         if (st->use_graphs) {
-            hipGraphExec_t ex = nullptr;
-            e = graph_for(st, mask, si, &ex);
-            if (e == hipSuccess)
-                e = hipGraphLaunch(ex, s);
+            e = replay_slot(st, si, mask, ntiles, s);
         } else {
-            e = launch_classes(st, mask, sl.d, s);
+            e = launch_classes(st, mask, sl.d, s, ntiles);
         }
         if (e == hipSuccess)
             e = hipEventRecord(sl.done, s);
@@ -318,23 +391,22 @@ int batch_config(ostft_ctx* ctx, int exp, int blocks_per_cu, int use_graphs, int
     rc = class_info(ctx, st, c);
     if (rc != OSTFT_OK)
         return rc;
-    if (blocks_per_cu > 0 && blocks_per_cu != st->per_cu[c]) {
-        // captured launches carry the grid: wait for work in flight, recapture
-        for (auto& sl : st->slot)
-            if (sl.used)
-                (void)hipEventSynchronize(sl.done);
-        st->drop_graphs();
+    // captured launches take a new setting at their next replay
+    if (blocks_per_cu > 0) {
+        st->persistent[c] = 1;
         st->per_cu[c] = std::min(blocks_per_cu, st->info[c].max_blocks_per_cu);
+    } else if (blocks_per_cu < 0) {
+        st->persistent[c] = 0;
     }
     if (current)
-        *current = (int)(class_grid(st, c) / (uint32_t)st->cu_count);
+        *current = st->persistent[c] ? (int)(class_grid(st, c, 0) / (uint32_t)st->cu_count) : 0;
     return OSTFT_OK;
 }
 
-// Times the persistent launch of one class over a synthetic ragged batch
-// (`samples` samples in requests of 1x..16x the window, overlap 0.5, Hann,
-// DC removal, compat power) for each workgroups-per-CU setting and keeps the
-// fastest.
+// Times one class over a synthetic ragged batch (`samples` samples in 32
+// requests, overlap 0.5, Hann, DC removal, compat power) in the dispatch form
+// and in the persistent form at each workgroups-per-CU setting, and keeps the
+// fastest (0 = dispatch).
 int autotune_batch(ostft_ctx* ctx, int exp, uint64_t samples, int* best_blocks_per_cu, double* best_ms)
 {
     BatchState* st = nullptr;
@@ -350,16 +422,13 @@ int autotune_batch(ostft_ctx* ctx, int exp, uint64_t samples, int* best_blocks_p
     const uint64_t N = 1ull << exp;
     if (samples < N)
         samples = N;
-    // ragged request lengths from a fixed LCG
+    // 32 requests of 1/2 .. 3/2 x samples/32 each (ragged, not window
+    // multiples): few enough that the host's preparation of a batch stays
+    // shorter than the GPU's work on the previous one
     std::vector<uint64_t> lens;
-    uint64_t total = 0, lcg = 0x5EED0005ull;
-    while (total < samples) {
-        lcg = lcg * 6364136223846793005ull + 1442695040888963407ull;
-        uint64_t L = N * (1 + (lcg >> 60)) + ((lcg >> 20) % N);
-        L = std::min(L, samples - total < N ? N : samples - total);
-        lens.push_back(L);
-        total += L;
-    }
+    const uint64_t per = std::max<uint64_t>(samples / 32, N);
+    for (int i = 0; i < 32; ++i)
+        lens.push_back(per / 2 + (per * (uint64_t)(i % 8)) / 8 + (uint64_t)(i * 37) % N);
     ostft_params p;
     std::memset(&p, 0, sizeof(p));
     p.exp = exp;
@@ -404,30 +473,35 @@ int autotune_batch(ostft_ctx* ctx, int exp, uint64_t samples, int* best_blocks_p
             j.in = din + reinterpret_cast<uint64_t>(j.in);
             j.out = dout + reinterpret_cast<uint64_t>(j.out);
         }
-        const int saved_graphs = st->use_graphs, saved_pc = st->per_cu[c];
+        const int saved_graphs = st->use_graphs, saved_pc = st->per_cu[c], saved_ps = st->persistent[c];
         st->use_graphs = 0;
         const int mx = st->info[c].max_blocks_per_cu;
-        std::vector<int> cand;
+        std::vector<int> cand{0};  // 0: one workgroup per tile; k > 0: persistent, k per CU
         for (int k = 1; k < mx; k *= 2)
             cand.push_back(k);
         cand.push_back(mx);
-        int best = mx;
+        int best = 0;
         float best_t = 1e30f;
         for (int k : cand) {
+            st->persistent[c] = k > 0;
             st->per_cu[c] = k;
+            // run; e0; run; e1: the second batch is prepared while the first
+            // runs, so e0 -> e1 is the GPU's time for one batch
             float tmin = 1e30f;
-            for (int r = 0; r < 4 && rc == OSTFT_OK && e == hipSuccess; ++r) {
-                e = hipEventRecord(e0, s);
-                if (e == hipSuccess)
+            for (int r = 0; r < 3 && rc == OSTFT_OK && e == hipSuccess; ++r) {
+                rc = run_batch(ctx, jobs.data(), (uint32_t)jobs.size(), s);
+                if (rc == OSTFT_OK)
+                    e = hipEventRecord(e0, s);
+                if (rc == OSTFT_OK && e == hipSuccess)
                     rc = run_batch(ctx, jobs.data(), (uint32_t)jobs.size(), s);
-                if (e == hipSuccess && rc == OSTFT_OK)
+                if (rc == OSTFT_OK && e == hipSuccess)
                     e = hipEventRecord(e1, s);
-                if (e == hipSuccess && rc == OSTFT_OK)
+                if (rc == OSTFT_OK && e == hipSuccess)
                     e = hipEventSynchronize(e1);
                 float ms = 0.0f;
-                if (e == hipSuccess && rc == OSTFT_OK)
+                if (rc == OSTFT_OK && e == hipSuccess)
                     e = hipEventElapsedTime(&ms, e0, e1);
-                if (r > 0)  // the first run warms caches and tables
+                if (r > 0)  // the first round warms caches and tables
                     tmin = std::min(tmin, ms);
             }
             if (rc != OSTFT_OK || e != hipSuccess)
@@ -439,11 +513,10 @@ int autotune_batch(ostft_ctx* ctx, int exp, uint64_t samples, int* best_blocks_p
         }
         st->use_graphs = saved_graphs;
         st->per_cu[c] = saved_pc;
+        st->persistent[c] = saved_ps;
         if (rc == OSTFT_OK && e == hipSuccess) {
-            for (auto& sl : st->slot)
-                if (sl.used)
-                    (void)hipEventSynchronize(sl.done);
-            st->drop_graphs();
+            // captured launches take it at their next replay
+            st->persistent[c] = best > 0;
             st->per_cu[c] = best;
             if (best_blocks_per_cu)
                 *best_blocks_per_cu = best;

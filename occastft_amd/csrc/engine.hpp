@@ -60,10 +60,13 @@ struct BatchClassArgs {
 struct BatchClassInfo {
     int frames_per_tile;    // G
     int threads;            // workgroup size
-    int max_blocks_per_cu;  // occupancy limit of the batch kernel
+    int max_blocks_per_cu;  // occupancy limit of the persistent batch kernel
 };
-hipError_t launch_batch(int log_m, const BatchClassArgs& b, uint32_t grid, hipStream_t s);
+// persistent: a fixed grid claiming tiles from the class counter; otherwise
+// one workgroup per tile (grid = the class's tile count)
+hipError_t launch_batch(int log_m, const BatchClassArgs& b, uint32_t grid, bool persistent, hipStream_t s);
 hipError_t batch_class_info(int log_m, BatchClassInfo* info);
+const void* batch_kernel_fn(int log_m, bool persistent);  // the kernel's host handle (graph nodes name it)
 
 // FP64 table generators (run once per table).
 // tw[j] = exp(-2 pi i j / m) for j < count (count <= m)

@@ -42,7 +42,7 @@ namespace ostft {
 // spills, else 1 (the compiler's choice).
 template <int LOG_M>
 struct BatchWaves {
-    static constexpr int value = (LOG_M == 9 || LOG_M == 13) ? 1 : 4;
+    static constexpr int value = (LOG_M == 9 || LOG_M == 11 || LOG_M == 13) ? 1 : 4;
 };
 
 // FusedArgs as the batch kernel reads it from the descriptor block: the same
@@ -306,10 +306,37 @@ __global__ void __launch_bounds__(Cfg<LOG_M>::WG) stft_fused_kernel(const FusedA
     fused_frames<LOG_M>(a, (uint64_t)xcd_block(blockIdx.x, gridDim.x) * C::G, lds, red, threadIdx.x);
 }
 
-// Persistent form for a batch of requests of one window size: every
-// workgroup claims tiles (G frames of one request) from the class's counter
-// until the tiles run out.  tile_job[t] names the request of tile t; the
-// request's first tile is jobs[j].tile_begin.
+// Batch of requests of one window size, two forms over the same tile queue
+// (tile t = G frames of request tile_job[t], whose first tile is
+// jobs[j].tile_begin):
+//   * dispatch: one workgroup per tile, the hardware dispatcher walks the
+//     queue (XCD-aware order);
+//   * persistent: a fixed grid whose workgroups claim tiles from the class's
+//     counter until the tiles run out.
+// On gfx9 one wave's vector loads and stores retire in order, so a persistent
+// workgroup's wait for its next tile's loads also waits for the acknowledgement
+// of its previous tile's stores; a dispatched workgroup simply ends and leaves
+// its stores to drain.  Which form is faster for a window size is measured
+// (ostft_autotune_batch).
+template <int LOG_M>
+__global__ void __launch_bounds__(Cfg<LOG_M>::WG)
+    stft_batch_dispatch_kernel(const char* __restrict__ blk, const uint32_t header_off)
+{
+    using C = Cfg<LOG_M>;
+    __shared__ float2 lds[C::G * C::PADM];
+    __shared__ float2 red[C::WG / 64];
+    const BatchClassHeader* h = reinterpret_cast<const BatchClassHeader*>(blk + header_off);
+    const uint32_t ntiles = h->ntiles;
+    // the grid is ntiles (or 1 for an empty class): a permutation of the tiles
+    const uint32_t tile = xcd_block(blockIdx.x, gridDim.x);
+    if (tile >= ntiles)
+        return;
+    const BatchJob* jobs = reinterpret_cast<const BatchJob*>(blk + h->jobs_off);
+    const uint32_t j = reinterpret_cast<const uint32_t*>(blk + h->tiles_off)[tile];
+    const FusedArgs a = from_global(*reinterpret_cast<const FusedArgsGlobal*>(&jobs[j].a));
+    fused_frames<LOG_M>(a, (uint64_t)(tile - jobs[j].tile_begin) * C::G, lds, red, threadIdx.x);
+}
+
 template <int LOG_M>
 __global__ void __launch_bounds__(Cfg<LOG_M>::WG) __attribute__((amdgpu_waves_per_eu(BatchWaves<LOG_M>::value)))
     stft_batch_kernel(const char* __restrict__ blk, uint32_t* __restrict__ counter, const uint32_t header_off)
@@ -324,13 +351,18 @@ __global__ void __launch_bounds__(Cfg<LOG_M>::WG) __attribute__((amdgpu_waves_pe
     const uint32_t ntiles = h->ntiles;
     const BatchJob* jobs = reinterpret_cast<const BatchJob*>(blk + h->jobs_off);
     const uint32_t* tile_job = reinterpret_cast<const uint32_t*>(blk + h->tiles_off);
+    if (threadIdx.x == 0)
+        claim = atomicAdd(counter, 1u);
+    __syncthreads();
     for (;;) {
-        if (threadIdx.x == 0)
-            claim = atomicAdd(counter, 1u);
-        __syncthreads();
         const uint32_t tile = __builtin_amdgcn_readfirstlane(claim);
         if (tile >= ntiles)
             break;  // uniform: every thread read the same claim
+        // claim the next tile now: the atomic's round trip overlaps this
+        // tile's loads instead of stalling the workgroup between tiles
+        uint32_t next = 0;
+        if (threadIdx.x == 0)
+            next = atomicAdd(counter, 1u);
         const uint32_t j = tile_job[tile];
         const FusedArgs a = from_global(*reinterpret_cast<const FusedArgsGlobal*>(&jobs[j].a));
         // the thread index, re-stated opaquely every tile: otherwise the
@@ -340,7 +372,11 @@ __global__ void __launch_bounds__(Cfg<LOG_M>::WG) __attribute__((amdgpu_waves_pe
         uint32_t tid = threadIdx.x;
         asm volatile("" : "+v"(tid));
         fused_frames<LOG_M>(a, (uint64_t)(tile - jobs[j].tile_begin) * C::G, lds, red, tid);
-        __syncthreads();  // LDS and `claim` are reused by the next tile
+        // every thread read `claim` before the body's first barrier, so it
+        // can be overwritten now; the barrier publishes it and frees the LDS
+        if (threadIdx.x == 0)
+            claim = next;
+        __syncthreads();
     }
 }
 
@@ -358,14 +394,19 @@ static hipError_t launch_t(const FusedArgs& a, hipStream_t s)
 }
 
 template <int LOG_M>
-static hipError_t launch_batch_t(const BatchClassArgs& b, uint32_t grid, hipStream_t s)
+static hipError_t launch_batch_t(const BatchClassArgs& b, uint32_t grid, bool persistent, hipStream_t s)
 {
     if (grid == 0)
         return hipErrorInvalidValue;
     char* blk = static_cast<char*>(b.base);
-    uint32_t* counter = reinterpret_cast<uint32_t*>(blk + b.header_off + offsetof(BatchClassHeader, counter));
-    hipLaunchKernelGGL(stft_batch_kernel<LOG_M>, dim3(grid), dim3(Cfg<LOG_M>::WG), 0, s,
-                       static_cast<const char*>(blk), counter, (uint32_t)b.header_off);
+    if (persistent) {
+        uint32_t* counter = reinterpret_cast<uint32_t*>(blk + b.header_off + offsetof(BatchClassHeader, counter));
+        hipLaunchKernelGGL(stft_batch_kernel<LOG_M>, dim3(grid), dim3(Cfg<LOG_M>::WG), 0, s,
+                           static_cast<const char*>(blk), counter, (uint32_t)b.header_off);
+    } else {
+        hipLaunchKernelGGL(stft_batch_dispatch_kernel<LOG_M>, dim3(grid), dim3(Cfg<LOG_M>::WG), 0, s,
+                           static_cast<const char*>(blk), (uint32_t)b.header_off);
+    }
     return hipGetLastError();
 }
 
@@ -381,21 +422,44 @@ static hipError_t batch_info_t(BatchClassInfo* info)
     return e;
 }
 
+template <int LOG_M>
+static const void* batch_fn_t(bool persistent)
+{
+    return persistent ? reinterpret_cast<const void*>(&stft_batch_kernel<LOG_M>)
+                      : reinterpret_cast<const void*>(&stft_batch_dispatch_kernel<LOG_M>);
+}
+
 bool fused_supported(int log_m) { return log_m >= 5 && log_m <= 13; }
 
-hipError_t launch_batch(int log_m, const BatchClassArgs& b, uint32_t grid, hipStream_t s)
+hipError_t launch_batch(int log_m, const BatchClassArgs& b, uint32_t grid, bool persistent, hipStream_t s)
 {
     switch (log_m) {
-    case 5: return launch_batch_t<5>(b, grid, s);
-    case 6: return launch_batch_t<6>(b, grid, s);
-    case 7: return launch_batch_t<7>(b, grid, s);
-    case 8: return launch_batch_t<8>(b, grid, s);
-    case 9: return launch_batch_t<9>(b, grid, s);
-    case 10: return launch_batch_t<10>(b, grid, s);
-    case 11: return launch_batch_t<11>(b, grid, s);
-    case 12: return launch_batch_t<12>(b, grid, s);
-    case 13: return launch_batch_t<13>(b, grid, s);
+    case 5: return launch_batch_t<5>(b, grid, persistent, s);
+    case 6: return launch_batch_t<6>(b, grid, persistent, s);
+    case 7: return launch_batch_t<7>(b, grid, persistent, s);
+    case 8: return launch_batch_t<8>(b, grid, persistent, s);
+    case 9: return launch_batch_t<9>(b, grid, persistent, s);
+    case 10: return launch_batch_t<10>(b, grid, persistent, s);
+    case 11: return launch_batch_t<11>(b, grid, persistent, s);
+    case 12: return launch_batch_t<12>(b, grid, persistent, s);
+    case 13: return launch_batch_t<13>(b, grid, persistent, s);
     default: return hipErrorInvalidValue;
+    }
+}
+
+const void* batch_kernel_fn(int log_m, bool persistent)
+{
+    switch (log_m) {
+    case 5: return batch_fn_t<5>(persistent);
+    case 6: return batch_fn_t<6>(persistent);
+    case 7: return batch_fn_t<7>(persistent);
+    case 8: return batch_fn_t<8>(persistent);
+    case 9: return batch_fn_t<9>(persistent);
+    case 10: return batch_fn_t<10>(persistent);
+    case 11: return batch_fn_t<11>(persistent);
+    case 12: return batch_fn_t<12>(persistent);
+    case 13: return batch_fn_t<13>(persistent);
+    default: return nullptr;
     }
 }
 

@@ -351,11 +351,19 @@ int ostft_server_create(int32_t device, const ostft_server_config* config, ostft
         e = hipMalloc(&s->arena_d, cfg.arena_bytes);
     if (e == hipSuccess)
         e = hipHostMalloc(&s->arena_h, cfg.arena_bytes, hipHostMallocDefault);
+    if (e == hipSuccess) {
+        // touch the pinned mirror and move it both ways once, so the first
+        // requests through each part of the ring do not pay first-use costs
+        std::memset(s->arena_h, 0, cfg.arena_bytes);
+        e = hipMemcpy(s->arena_d, s->arena_h, cfg.arena_bytes, hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = hipMemcpy(s->arena_h, s->arena_d, cfg.arena_bytes, hipMemcpyDeviceToHost);
+    }
     if (e == hipSuccess)
         rc = ostft_batch_config(s->ctx, 0, 0, cfg.use_graphs ? 1 : 0, nullptr);
     if (e == hipSuccess && rc == OSTFT_OK && cfg.autotune) {
         for (int exp = 6; exp <= 14 && rc == OSTFT_OK; ++exp)
-            rc = ostft_autotune_batch(s->ctx, exp, 1ull << 24, nullptr, nullptr);
+            rc = ostft_autotune_batch(s->ctx, exp, 1ull << 26, nullptr, nullptr);
     }
     if (e != hipSuccess || rc != OSTFT_OK) {
         if (s->arena_d)

@@ -167,6 +167,9 @@ class Runner:
         return outs
 
     def batch_config(self, exp: int = 0, blocks_per_cu: int = 0, use_graphs: int = -1) -> int:
+        """blocks_per_cu > 0: persistent form with that many workgroups per CU;
+        < 0: one workgroup per tile; 0: unchanged.  Returns the setting in
+        force (0 = one workgroup per tile)."""
         cur = ctypes.c_int32()
         rc = N.lib().ostft_batch_config(self._h, int(exp), int(blocks_per_cu), int(use_graphs), ctypes.byref(cur))
         if rc != N.OK:
@@ -174,7 +177,8 @@ class Runner:
         return cur.value
 
     def autotune_batch(self, exp: int, samples: int = 1 << 24):
-        """-> (best workgroups per CU, best ms) for window exponent `exp`."""
+        """-> (best setting, best ms) for window exponent `exp`: 0 = one
+        workgroup per tile, k > 0 = persistent with k workgroups per CU."""
         best = ctypes.c_int32()
         ms = ctypes.c_double()
         rc = N.lib().ostft_autotune_batch(self._h, int(exp), int(samples), ctypes.byref(best), ctypes.byref(ms))

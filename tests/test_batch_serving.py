@@ -91,14 +91,21 @@ def bitwise_equal(a, b):
 # ------------------------------------------------------------------ batches
 
 @pytest.mark.parametrize("misalign", [False, True])
-def test_batch_matches_requests_run_alone(runner, misalign):
+@pytest.mark.parametrize("form", ["dispatch", "persistent"])
+def test_batch_matches_requests_run_alone(runner, misalign, form):
     import torch
 
     reqs = ragged_requests(11 + misalign, 48, exps=list(range(6, 15)) + [15, 16])
     alone = [run_alone(runner, x, p) for x, p in reqs]
     xs = device_inputs(reqs, misalign)
-    outs = runner.run_batch_device([(xd, p, None) for xd, (_, p) in zip(xs, reqs)])
-    torch.cuda.synchronize()
+    for e in range(6, 15):
+        runner.batch_config(e, blocks_per_cu=2 if form == "persistent" else -1)
+    try:
+        outs = runner.run_batch_device([(xd, p, None) for xd, (_, p) in zip(xs, reqs)])
+        torch.cuda.synchronize()
+    finally:
+        for e in range(6, 15):
+            runner.batch_config(e, blocks_per_cu=-1)
     for i, (o, a) in enumerate(zip(outs, alone)):
         assert bitwise_equal(o.cpu().numpy(), a), f"request {i}: exp {reqs[i][1].exp} out {reqs[i][1].output}"
 
@@ -138,10 +145,11 @@ def test_batch_graph_replays_match_direct_launches(runner):
             torch.cuda.synchronize()
             for i, (o, r) in enumerate(zip(outs, ref)):
                 assert bitwise_equal(o.cpu().numpy(), r), f"replay {rep} request {i}"
-            # eager work and a different batch in between
+            # eager work and a batch of another set of window sizes in between
             run_alone(runner, reqs[0][0], reqs[0][1])
-            runner.run_batch_device([(xs[1], reqs[1][1], None)])
+            one = runner.run_batch_device([(xs[1], reqs[1][1], None)])
             torch.cuda.synchronize()
+            assert bitwise_equal(one[0].cpu().numpy(), ref[1]), f"replay {rep}: single-request batch"
     finally:
         runner.batch_config(use_graphs=1)
 
@@ -171,8 +179,8 @@ def test_empty_batch(runner):
 def test_autotune_keeps_results(runner):
     import torch
 
-    best, ms = runner.autotune_batch(10, 1 << 22)
-    assert best >= 1 and ms > 0
+    best, ms = runner.autotune_batch(10, 1 << 24)
+    assert best >= 0 and ms > 0  # 0: one workgroup per tile
     assert runner.batch_config(10) == best
     reqs = ragged_requests(31, 12, exps=[10])
     alone = [run_alone(runner, x, p) for x, p in reqs]
@@ -181,7 +189,7 @@ def test_autotune_keeps_results(runner):
     torch.cuda.synchronize()
     for o, a in zip(outs, alone):
         assert bitwise_equal(o.cpu().numpy(), a)
-    for bpc in (1, 2):
+    for bpc in (1, 2, -1):  # persistent at 1 and 2 per CU, then one workgroup per tile
         runner.batch_config(10, blocks_per_cu=bpc)
         outs = runner.run_batch_device([(xd, p, None) for xd, (_, p) in zip(xs, reqs)])
         torch.cuda.synchronize()
