@@ -86,6 +86,12 @@ enum ostft_output {
 #define OSTFT_FLAG_SIGMA_EXACT 0x8u    /* parse "--gaussian_window=<s><<sigma" from the first
                                           character of <s>; without it the first character is
                                           skipped as in RunnerCommon.cpp:228 */
+#define OSTFT_FLAG_MFMA_FP16 0x10u     /* "--mfma_fp16": the transform as split-precision
+                                          FP16 GEMMs on the matrix cores (hi/lo operands,
+                                          3 products, FP32 accumulation); windows 64, 256,
+                                          1024 only */
+#define OSTFT_FLAG_MFMA_BF16 0x20u     /* "--mfma_bf16": the same with BF16 operands (16
+                                          significant bits: about 1e-5 relative error) */
 
 typedef struct ostft_params {
     int32_t exp;      /* window size N = 2^exp, 6 <= exp <= 20 */
@@ -113,7 +119,7 @@ typedef struct ostft_ctx ostft_ctx;
  * precedence).  exp/overlap are copied into the result.  New options, ignored
  * by the reference runners: "--magnitude_return", "--power_return",
  * "--half_spectrum_return", "--exact_odd_exp", "--exact_window",
- * "--exact_sigma_parse".  Returns OSTFT_OK or OSTFT_ERR_OPTION. */
+ * "--exact_sigma_parse", "--mfma_fp16", "--mfma_bf16".  Returns OSTFT_OK or OSTFT_ERR_OPTION. */
 int ostft_parse_options(const char* options, int32_t exp, float overlap, ostft_params* out);
 
 /* Frame geometry of SerialImpl.cpp:36-43 / RunnerInterface.hpp:26-35 in 64 bits. */
@@ -130,6 +136,15 @@ void ostft_destroy(ostft_ctx* ctx);
  * FP64-generated twiddle tables synchronously.  Replaces the device part of Runner::ActivateSTFT. */
 int ostft_run_device(ostft_ctx* ctx, const ostft_params* params, const float* d_in,
                      uint64_t full_size, float* d_out, uint64_t out_cap, void* hip_stream);
+
+/* Strided batch of `count` equal requests: signal y is d_in + y * in_stride
+ * (full_size floats), its result goes to d_out + y * out_stride (out_stride a
+ * multiple of 4 floats, at least the layout's out_floats).  One launch (for
+ * windows up to 2^14); results are bitwise those of ostft_run_device on each
+ * signal.  No counterpart in the reference (one signal per ActivateSTFT). */
+int ostft_run_strided_device(ostft_ctx* ctx, const ostft_params* params, const float* d_in,
+                             uint64_t full_size, uint64_t in_stride, uint32_t count, float* d_out,
+                             uint64_t out_stride, void* hip_stream);
 
 /* Host-buffer STFT with the reference's ActivateSTFT signature folded into C:
  * parses `options`, copies `in` to the device, runs, copies the result back

@@ -8,6 +8,8 @@ callers.
 from ._native import (  # noqa: F401
     FLAG_EXACT_ODD,
     FLAG_EXACT_WINDOW,
+    FLAG_MFMA_BF16,
+    FLAG_MFMA_FP16,
     FLAG_REMOVE_DC,
     FLAG_SIGMA_EXACT,
     OUT_COMPAT_POWER,

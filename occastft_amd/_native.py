@@ -35,6 +35,8 @@ FLAG_REMOVE_DC = 0x1
 FLAG_EXACT_ODD = 0x2
 FLAG_EXACT_WINDOW = 0x4
 FLAG_SIGMA_EXACT = 0x8
+FLAG_MFMA_FP16 = 0x10
+FLAG_MFMA_BF16 = 0x20
 
 # every symbol the header declares
 EXPORTS = (
@@ -49,6 +51,7 @@ EXPORTS = (
     "ostft_abi_version",
     "ostft_window_coefficients",
     "ostft_run_batch_device",
+    "ostft_run_strided_device",
     "ostft_batch_config",
     "ostft_autotune_batch",
     "ostft_server_create",
@@ -173,6 +176,9 @@ def lib() -> ctypes.CDLL:
     u64p = ctypes.POINTER(ctypes.c_uint64)
     L.ostft_run_batch_device.restype = ctypes.c_int
     L.ostft_run_batch_device.argtypes = [vp, ctypes.POINTER(Job), ctypes.c_uint32, vp]
+    L.ostft_run_strided_device.restype = ctypes.c_int
+    L.ostft_run_strided_device.argtypes = [vp, ctypes.POINTER(Params), vp, ctypes.c_uint64, ctypes.c_uint64,
+                                           ctypes.c_uint32, vp, ctypes.c_uint64, vp]
     L.ostft_batch_config.restype = ctypes.c_int
     L.ostft_batch_config.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, i32p]
     L.ostft_autotune_batch.restype = ctypes.c_int

@@ -279,6 +279,9 @@ int run_batch(ostft_ctx* ctx, const ostft_job* jobs, uint32_t njobs, hipStream_t
     for (uint32_t i = 0; i < njobs; ++i) {
         const ostft_job& j = jobs[i];
         rc = ctx_prepare(ctx, &j.params, j.in, j.full_size, j.out, j.out_cap, &args[i], &lm[i]);
+        if (rc == OSTFT_OK && (j.params.flags & (OSTFT_FLAG_MFMA_FP16 | OSTFT_FLAG_MFMA_BF16)))
+            rc = ctx_fail(ctx, OSTFT_ERR_UNSUPPORTED,
+                          "the MFMA form runs through ostft_run_device and ostft_run_strided_device");
         if (rc != OSTFT_OK)
             return ctx_fail(ctx, rc, "job " + std::to_string(i) + ": " + ostft_last_error(ctx));
     }

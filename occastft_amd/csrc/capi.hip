@@ -7,6 +7,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cerrno>
 #include <cmath>
 #include <cstdlib>
@@ -72,6 +73,15 @@ int fail(ostft_ctx* ctx, int code, const std::string& msg)
 
 bool contains(const char* s, const char* key) { return std::strstr(s, key) != nullptr; }
 
+int transform_of(const ostft_params* p)
+{
+    if (p->flags & OSTFT_FLAG_MFMA_FP16)
+        return ostft::XF_MFMA_FP16;
+    if (p->flags & OSTFT_FLAG_MFMA_BF16)
+        return ostft::XF_MFMA_BF16;
+    return ostft::XF_FP32;
+}
+
 // toQuot, StandAlone/cross_gpgpu/GLOBAL/RunnerInterface.hpp:26-35: float32 throughout.
 #pragma clang fp contract(off)
 double to_quot_f32(float fullSize, float overlapRatio, float windowSize)
@@ -108,6 +118,10 @@ int make_plan(const ostft_params* p, uint64_t L, Plan* plan, ostft_ctx* ctx)
         return fail(ctx, OSTFT_ERR_INVALID_ARG, "unknown window kind");
     if (p->output < OSTFT_OUT_COMPAT_POWER || p->output > OSTFT_OUT_HALF_SPECTRUM)
         return fail(ctx, OSTFT_ERR_INVALID_ARG, "unknown output layout");
+    if ((p->flags & OSTFT_FLAG_MFMA_FP16) && (p->flags & OSTFT_FLAG_MFMA_BF16))
+        return fail(ctx, OSTFT_ERR_INVALID_ARG, "choose one of the FP16 and BF16 MFMA forms");
+    if ((p->flags & (OSTFT_FLAG_MFMA_FP16 | OSTFT_FLAG_MFMA_BF16)) && !ostft::mfma_supported(p->exp - 1))
+        return fail(ctx, OSTFT_ERR_UNSUPPORTED, "the MFMA split-precision form covers windows 64, 256 and 1024");
     ostft_geometry& g = plan->g;
     const uint64_t N = 1ull << p->exp;
     const double q = to_quot_f32((float)L, p->overlap, (float)N);
@@ -294,6 +308,8 @@ int ctx_prepare(ostft_ctx* ctx, const ostft_params* params, const float* d_in, u
     a->combine = plan.combine;
     a->out_mode = params->output;
     a->dc = (params->flags & OSTFT_FLAG_REMOVE_DC) ? 1 : 0;
+    a->in_stride = 0;
+    a->out_stride = 0;
     a->tw = nullptr;
     a->tw_n = get_twiddles(ctx, 2 * M, M, &rc);  // W_N^k, k < M
     if (!a->tw_n)
@@ -358,6 +374,10 @@ int ostft_parse_options(const char* options, int32_t exp, float overlap, ostft_p
         p.flags |= OSTFT_FLAG_EXACT_WINDOW;
     if (contains(s, "--exact_sigma_parse"))
         p.flags |= OSTFT_FLAG_SIGMA_EXACT;
+    if (contains(s, "--mfma_fp16"))
+        p.flags |= OSTFT_FLAG_MFMA_FP16;
+    else if (contains(s, "--mfma_bf16"))
+        p.flags |= OSTFT_FLAG_MFMA_BF16;
     static const char* const keys[7] = {"--hanning_window",        "--hamming_window",
                                         "--blackman_window",       "--nuttall_window",
                                         "--blackman_nuttall_window", "--blackman_harris_window",
@@ -481,12 +501,58 @@ int ostft_run_device(ostft_ctx* ctx, const ostft_params* params, const float* d_
     if (a.frames == 0)
         return OSTFT_OK;
     if (ostft::fused_supported(log_m)) {
-        hipError_t e = ostft::launch_fused(log_m, a, s);
+        hipError_t e = ostft::launch_fused(log_m, transform_of(params), a, 1, s);
         if (e != hipSuccess)
             return hip_fail(ctx, e, "fused STFT launch");
         return OSTFT_OK;
     }
     return ostft::run_multipass(ctx, log_m, a, s);
+}
+
+int ostft_run_strided_device(ostft_ctx* ctx, const ostft_params* params, const float* d_in, uint64_t full_size,
+                             uint64_t in_stride, uint32_t count, float* d_out, uint64_t out_stride,
+                             void* hip_stream)
+{
+    if (!ctx)
+        return OSTFT_ERR_INVALID_ARG;
+    DeviceGuard dg(ctx->device);
+    if (!dg.ok)
+        return fail(ctx, OSTFT_ERR_NO_DEVICE, "hipSetDevice failed");
+    if (count == 0)
+        return OSTFT_OK;
+    ostft::FusedArgs a;
+    int log_m = 0;
+    const int rc = ostft::ctx_prepare(ctx, params, d_in, full_size, d_out, out_stride, &a, &log_m);
+    if (rc != OSTFT_OK)
+        return rc;
+    if (out_stride % 4 != 0)
+        return fail(ctx, OSTFT_ERR_INVALID_ARG, "out_stride must be a multiple of 4 floats");
+    hipStream_t s = static_cast<hipStream_t>(hip_stream);
+    if (a.frames == 0)
+        return OSTFT_OK;
+    a.in_stride = in_stride;
+    a.out_stride = out_stride;
+    if (ostft::fused_supported(log_m)) {
+        for (uint32_t y0 = 0; y0 < count; y0 += 65535u) {  // grid.y limit
+            ostft::FusedArgs ay = a;
+            ay.in = d_in + (uint64_t)y0 * in_stride;
+            ay.out = d_out + (uint64_t)y0 * out_stride;
+            const hipError_t e = ostft::launch_fused(log_m, transform_of(params), ay, std::min(count - y0, 65535u), s);
+            if (e != hipSuccess)
+                return hip_fail(ctx, e, "strided STFT launch");
+        }
+        return OSTFT_OK;
+    }
+    for (uint32_t y = 0; y < count; ++y) {
+        ostft::FusedArgs ay = a;
+        ay.in = d_in + (uint64_t)y * in_stride;
+        ay.out = d_out + (uint64_t)y * out_stride;
+        ay.in_stride = ay.out_stride = 0;
+        const int r = ostft::run_multipass(ctx, log_m, ay, s);
+        if (r != OSTFT_OK)
+            return r;
+    }
+    return OSTFT_OK;
 }
 
 int ostft_run_batch_device(ostft_ctx* ctx, const ostft_job* jobs, uint32_t njobs, void* hip_stream)

@@ -33,11 +33,19 @@ struct FusedArgs {
     int32_t combine;      // 1: full FFT; 0: stop one radix-2 stage early (odd-exponent result)
     int32_t out_mode;     // OutMode
     int32_t dc;           // subtract the frame mean after windowing
+    uint64_t in_stride;   // strided batches: floats between signal y's input and y+1's
+    uint64_t out_stride;  //                  and between their outputs (blockIdx.y = y)
 };
+
+// transform of the fused kernel: Stockham FFT in FP32, or the split-precision
+// MFMA DFT (windows 64, 256, 1024)
+enum Transform : int { XF_FP32 = 0, XF_MFMA_FP16 = 1, XF_MFMA_BF16 = 2 };
+bool mfma_supported(int log_m);
 
 // Single-kernel path: complex size M = 2^log_m, 5 <= log_m <= 13 (N = 64 .. 16384).
 bool fused_supported(int log_m);
-hipError_t launch_fused(int log_m, const FusedArgs& a, hipStream_t s);
+// `count` signals (grid.y), signal y at a.in + y in_stride / a.out + y out_stride
+hipError_t launch_fused(int log_m, int xf, const FusedArgs& a, uint32_t count, hipStream_t s);
 
 // ---- batches: many requests of one window size in one persistent launch ----
 // A batch descriptor block lives in device memory (uploaded by the host before

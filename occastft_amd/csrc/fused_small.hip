@@ -27,6 +27,7 @@
 //   * Blocks are renumbered so each XCD walks a contiguous range of frames.
 #include "engine.hpp"
 #include "epilogue.hpp"
+#include "mfma_dft.hpp"
 #include "stockham.hpp"
 
 namespace ostft {
@@ -59,6 +60,7 @@ struct FusedArgsGlobal {
     float OSTFT_GLOBAL* out;
     uint64_t out_limit;
     int32_t combine, out_mode, dc;
+    uint64_t in_stride, out_stride;
 };
 static_assert(sizeof(FusedArgsGlobal) == sizeof(FusedArgs), "FusedArgsGlobal mirrors FusedArgs");
 static_assert(offsetof(FusedArgsGlobal, out) == offsetof(FusedArgs, out), "FusedArgsGlobal mirrors FusedArgs");
@@ -79,6 +81,8 @@ __device__ __forceinline__ FusedArgs from_global(const FusedArgsGlobal& g)
     a.combine = g.combine;
     a.out_mode = g.out_mode;
     a.dc = g.dc;
+    a.in_stride = g.in_stride;
+    a.out_stride = g.out_stride;
     return a;
 }
 
@@ -89,7 +93,7 @@ __device__ __forceinline__ uint32_t spad(uint32_t q) { return q + ((q >> 7) << 2
 // The work of one workgroup: frames f0 .. f0+G-1 of request `a` (fewer at the
 // end of the request).  Shared by the one-request kernel and the persistent
 // batch kernel, so a frame's result does not depend on which of them ran it.
-template <int LOG_M>
+template <int LOG_M, int XF = XF_FP32>
 __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t f0, float2* lds, float2* red,
                                              const uint32_t tid)
 {
@@ -145,7 +149,8 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
             wv[m] = w2[t + m * T];
     }
     PassTwiddles<LOG_M> ptw;
-    load_pass_twiddles<LOG_M>(ptw, a.tw, t);
+    if constexpr (XF == XF_FP32)
+        load_pass_twiddles<LOG_M>(ptw, a.tw, t);
     // epilogue: the thread handles BPT consecutive bins per iteration, BPT = 4
     // for the real layouts (one float4 per half) and 2 for the complex layout
     // (one float4 of two complex values per half), so that every store
@@ -223,7 +228,16 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
     }
 
     // ---- 4. FFT of the packed frame; Z stays in LDS, natural order ----
-    stockham<LOG_M, 0>(v, lds + grp * PADM, ptw, t);
+    if constexpr (XF == XF_FP32) {
+        stockham<LOG_M, 0>(v, lds + grp * PADM, ptw, t);
+    } else {
+        float2* z = lds + grp * PADM;
+#pragma unroll
+        for (int m = 0; m < 16; ++m)
+            z[pad(t + m * T)] = v[m];
+        __syncthreads();
+        mfma_fft<LOG_M, M, G, PADM, WG, XF>(lds, a.tw, tid);
+    }
 
     // ---- 5. last radix-2 stage + output, the whole block's output at once ----
     float* __restrict__ out = a.out;
@@ -297,13 +311,17 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
     }
 }
 
-template <int LOG_M>
+// One request, or a strided batch of equal requests (signal y = blockIdx.y).
+template <int LOG_M, int XF>
 __global__ void __launch_bounds__(Cfg<LOG_M>::WG) stft_fused_kernel(const FusedArgs a)
 {
     using C = Cfg<LOG_M>;
     __shared__ float2 lds[C::G * C::PADM];
     __shared__ float2 red[C::WG / 64];
-    fused_frames<LOG_M>(a, (uint64_t)xcd_block(blockIdx.x, gridDim.x) * C::G, lds, red, threadIdx.x);
+    FusedArgs ay = a;
+    ay.in += (uint64_t)blockIdx.y * a.in_stride;
+    ay.out += (uint64_t)blockIdx.y * a.out_stride;
+    fused_frames<LOG_M, XF>(ay, (uint64_t)xcd_block(blockIdx.x, gridDim.x) * C::G, lds, red, threadIdx.x);
 }
 
 // Batch of requests of one window size, two forms over the same tile queue
@@ -380,18 +398,34 @@ __global__ void __launch_bounds__(Cfg<LOG_M>::WG) __attribute__((amdgpu_waves_pe
     }
 }
 
-template <int LOG_M>
-static hipError_t launch_t(const FusedArgs& a, hipStream_t s)
+template <int LOG_M, int XF>
+static hipError_t launch_t(const FusedArgs& a, uint32_t count, hipStream_t s)
 {
     using C = Cfg<LOG_M>;
     const uint64_t blocks = (a.frames + C::G - 1) / C::G;
-    if (blocks == 0)
+    if (blocks == 0 || count == 0)
         return hipSuccess;
-    if (blocks > 0x7fffffffull)
+    if (blocks > 0x7fffffffull || count > 65535u)
         return hipErrorInvalidValue;
-    hipLaunchKernelGGL(stft_fused_kernel<LOG_M>, dim3((uint32_t)blocks), dim3(C::WG), 0, s, a);
+    hipLaunchKernelGGL((stft_fused_kernel<LOG_M, XF>), dim3((uint32_t)blocks, count), dim3(C::WG), 0, s, a);
     return hipGetLastError();
 }
+
+template <int LOG_M>
+static hipError_t launch_xf(int xf, const FusedArgs& a, uint32_t count, hipStream_t s)
+{
+    if constexpr (LOG_M == 5 || LOG_M == 7 || LOG_M == 9) {
+        if (xf == XF_MFMA_FP16)
+            return launch_t<LOG_M, XF_MFMA_FP16>(a, count, s);
+        if (xf == XF_MFMA_BF16)
+            return launch_t<LOG_M, XF_MFMA_BF16>(a, count, s);
+    }
+    if (xf != XF_FP32)
+        return hipErrorInvalidValue;
+    return launch_t<LOG_M, XF_FP32>(a, count, s);
+}
+
+bool mfma_supported(int log_m) { return log_m == 5 || log_m == 7 || log_m == 9; }
 
 template <int LOG_M>
 static hipError_t launch_batch_t(const BatchClassArgs& b, uint32_t grid, bool persistent, hipStream_t s)
@@ -479,18 +513,18 @@ hipError_t batch_class_info(int log_m, BatchClassInfo* info)
     }
 }
 
-hipError_t launch_fused(int log_m, const FusedArgs& a, hipStream_t s)
+hipError_t launch_fused(int log_m, int xf, const FusedArgs& a, uint32_t count, hipStream_t s)
 {
     switch (log_m) {
-    case 5: return launch_t<5>(a, s);
-    case 6: return launch_t<6>(a, s);
-    case 7: return launch_t<7>(a, s);
-    case 8: return launch_t<8>(a, s);
-    case 9: return launch_t<9>(a, s);
-    case 10: return launch_t<10>(a, s);
-    case 11: return launch_t<11>(a, s);
-    case 12: return launch_t<12>(a, s);
-    case 13: return launch_t<13>(a, s);
+    case 5: return launch_xf<5>(xf, a, count, s);
+    case 6: return launch_xf<6>(xf, a, count, s);
+    case 7: return launch_xf<7>(xf, a, count, s);
+    case 8: return launch_xf<8>(xf, a, count, s);
+    case 9: return launch_xf<9>(xf, a, count, s);
+    case 10: return launch_xf<10>(xf, a, count, s);
+    case 11: return launch_xf<11>(xf, a, count, s);
+    case 12: return launch_xf<12>(xf, a, count, s);
+    case 13: return launch_xf<13>(xf, a, count, s);
     default: return hipErrorInvalidValue;
     }
 }

@@ -132,6 +132,29 @@ class Runner:
             raise STFTError(rc, self.last_error)
         return out[: g["out_floats"]]
 
+    def run_strided_device(self, x, params: N.Params, out=None, stream=None):
+        """STFT of every row of a (count, L) float32 CUDA tensor in one launch
+        (ostft_run_strided_device); returns a (count, out_floats) view of
+        `out` (allocated with rows padded to 4 floats if None)."""
+        import torch
+
+        if x.dtype != torch.float32 or not x.is_cuda or x.dim() != 2 or x.stride(1) != 1:
+            raise ValueError("x must be a (count, L) float32 CUDA tensor with contiguous rows")
+        count, L = x.shape
+        g = geometry(params, L)
+        row = (g["out_floats"] + 3) // 4 * 4
+        if out is None:
+            out = torch.empty(count, row, dtype=torch.float32, device=x.device)
+        if out.dim() != 2 or out.shape[0] != count or out.stride(1) != 1:
+            raise ValueError("out must be (count, >= out_floats) with contiguous rows")
+        s = stream if stream is not None else torch.cuda.current_stream(x.device)
+        rc = N.lib().ostft_run_strided_device(self._h, ctypes.byref(params), ctypes.c_void_p(x.data_ptr()), L,
+                                              x.stride(0), count, ctypes.c_void_p(out.data_ptr()), out.stride(0),
+                                              ctypes.c_void_p(s.cuda_stream))
+        if rc != N.OK:
+            raise STFTError(rc, self.last_error)
+        return out[:, : g["out_floats"]]
+
     # -- batches -----------------------------------------------------------
     def run_batch_device(self, jobs, stream=None):
         """Run [(x, params, out), ...] of float32 CUDA tensors in one batch

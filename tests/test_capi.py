@@ -90,6 +90,21 @@ def test_new_options():
     assert p.sigma == pytest.approx(2.5)
 
 
+def test_mfma_options_and_limits():
+    p = ostft.parse_options("--hanning_window --mfma_fp16", 10, 0.5)
+    assert p.flags & N.FLAG_MFMA_FP16 and not p.flags & N.FLAG_MFMA_BF16
+    assert ostft.parse_options("--mfma_bf16", 8, 0.5).flags & N.FLAG_MFMA_BF16
+    for exp in (6, 8, 10):
+        ostft.geometry(ostft.make_params(exp, 0.5, flags=N.FLAG_MFMA_FP16), 5000)
+    for exp in (7, 9, 11, 12, 16):
+        with pytest.raises(ostft.STFTError) as ei:
+            ostft.geometry(ostft.make_params(exp, 0.5, flags=N.FLAG_MFMA_BF16), 5000)
+        assert ei.value.status == N.ERR_UNSUPPORTED
+    with pytest.raises(ostft.STFTError) as ei:
+        ostft.geometry(ostft.make_params(10, 0.5, flags=N.FLAG_MFMA_FP16 | N.FLAG_MFMA_BF16), 5000)
+    assert ei.value.status == N.ERR_INVALID_ARG
+
+
 def test_geometry_matches_oracle(rng):
     for _ in range(500):
         L = int(rng.integers(0, 1 << 30))

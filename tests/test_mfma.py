@@ -1,0 +1,148 @@
+"""The split-precision MFMA form (--mfma_fp16 / --mfma_bf16) and strided
+batches.
+
+Bar for the FP16 form: per-frame relative L2 error <= 1e-5 against the FP64
+pipeline (oracle/fp64.c), the same bar as the FP32 path, for every window kind
+and output layout on windows 64, 256, 1024, including signals scaled by 1e-20
+and 1e20 (every GEMM column is scaled by a power of two from its own maximum)
+and all-zero signals.  The BF16 form carries 16 significant bits per operand
+pair: its error against FP64 is measured and bounded at 1e-4, not 1e-5.
+"""
+import numpy as np
+import pytest
+
+from conftest import frame_errors
+
+import occastft_amd as ostft
+from occastft_amd import _native as N
+
+import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+EXPS = (6, 8, 10)
+FP64_MODE = {N.OUT_COMPAT_POWER: po.OUT_COMPAT_POWER, N.OUT_HALF_COMPLEX: po.OUT_HALF_COMPLEX,
+             N.OUT_MAGNITUDE: po.OUT_MAGNITUDE, N.OUT_POWER: po.OUT_POWER,
+             N.OUT_HALF_SPECTRUM: po.OUT_HALF_SPECTRUM}
+WINDOW_NAME = {N.WIN_NONE: "none", N.WIN_HANNING: "hanning", N.WIN_HAMMING: "hamming",
+               N.WIN_BLACKMAN: "blackman", N.WIN_BLACKMAN_HARRIS: "blackman_harris"}
+# against FP64 the window is the FP64 formula (--exact_window): the reference's
+# float coefficients differ from it by up to 4e-4 near the tapers, which is not
+# what these tests measure
+EXACT = N.FLAG_EXACT_WINDOW
+
+
+@pytest.fixture(scope="module")
+def runner():
+    import torch
+
+    assert torch.cuda.is_available(), "GPU tests need a GPU"
+    r = ostft.Runner(0)
+    yield r
+    r.close()
+
+
+def run(runner, x, p):
+    import torch
+
+    y = runner.run_device(torch.from_numpy(x).cuda(), p)
+    torch.cuda.synchronize()
+    return y.cpu().numpy()
+
+
+def fp64(x, p):
+    g = ostft.geometry(p, x.size)
+    win = po.WINDOWS[WINDOW_NAME[p.window]]
+    frames = g["frames"]
+    return po.stft64(x, p.exp, g["hop"], frames, win, 0.0, bool(p.flags & N.FLAG_REMOVE_DC),
+                     FP64_MODE[p.output]), g
+
+
+def per_frame(p):
+    N_ = 1 << p.exp
+    return {N.OUT_POWER: N_ // 2 + 1, N.OUT_HALF_SPECTRUM: N_ + 2}.get(p.output, N_)
+
+
+def check_vs_fp64(runner, x, p, tol):
+    y = run(runner, x, p)
+    r, g = fp64(x, p)
+    r = np.asarray(r, np.float64)[: y.size]
+    assert y.size == g["out_floats"]
+    mx, p50, zabs = frame_errors(y, r, per_frame(p))
+    if p.output == N.OUT_POWER:
+        tol = 2 * tol  # |X|^2 doubles the relative error of |X|
+    assert mx <= tol and zabs <= 1e-6 * max(1.0, float(np.abs(x).max())), \
+        (p.exp, p.window, p.output, float(np.abs(x).max()), mx)
+    return mx
+
+
+@pytest.mark.parametrize("exp", EXPS)
+@pytest.mark.parametrize("out", sorted(FP64_MODE))
+def test_fp16_form_against_fp64(runner, exp, out):
+    x = np.random.default_rng(exp * 10 + out).uniform(-1, 1, 40000).astype(np.float32)
+    for win in (N.WIN_NONE, N.WIN_HANNING, N.WIN_BLACKMAN_HARRIS):
+        p = ostft.make_params(exp, 0.5, win, 0.0, out, N.FLAG_REMOVE_DC | N.FLAG_MFMA_FP16 | EXACT)
+        check_vs_fp64(runner, x, p, 1e-5)
+
+
+@pytest.mark.parametrize("exp", EXPS)
+def test_fp16_form_is_scale_invariant(runner, exp):
+    """Columns are scaled by powers of two: tiny and huge signals keep the
+    same relative error (no FP16 underflow or overflow).  The complex layout
+    is used: magnitudes square the values, which leaves the FP32 range at
+    1e+-20 on any path."""
+    base = np.random.default_rng(3).standard_normal(30000)
+    for scale in (1e-20, 1e-6, 1.0, 1e6, 1e20):
+        x = (base * scale).astype(np.float32)
+        p = ostft.make_params(exp, 0.75, N.WIN_HAMMING, 0.0, N.OUT_HALF_SPECTRUM, N.FLAG_MFMA_FP16 | EXACT)
+        check_vs_fp64(runner, x, p, 1e-5)
+
+
+@pytest.mark.parametrize("exp", EXPS)
+def test_fp16_form_pure_tones_and_zeros(runner, exp):
+    n = np.arange(50000)
+    N_ = 1 << exp
+    x = (np.cos(2 * np.pi * 5 * n / N_) + 1e-3 * np.sin(2 * np.pi * 0.31 * n)).astype(np.float32)
+    x[20000:30000] = 0.0  # all-zero frames
+    p = ostft.make_params(exp, 0.5, N.WIN_HANNING, 0.0, N.OUT_POWER, N.FLAG_REMOVE_DC | N.FLAG_MFMA_FP16 | EXACT)
+    check_vs_fp64(runner, x, p, 1e-5)
+
+
+@pytest.mark.parametrize("exp", EXPS)
+def test_bf16_form_error_is_bounded(runner, exp):
+    x = np.random.default_rng(exp).uniform(-1, 1, 40000).astype(np.float32)
+    p = ostft.make_params(exp, 0.5, N.WIN_HANNING, 0.0, N.OUT_MAGNITUDE,
+                          N.FLAG_REMOVE_DC | N.FLAG_MFMA_BF16 | EXACT)
+    mx = check_vs_fp64(runner, x, p, 1e-4)
+    assert mx > 1e-7  # it is the 16-bit form, not the FP32 path
+
+
+def test_mfma_form_against_the_serial_runner(runner):
+    """Default layout (the Serial runner's sqrt(2)|Re X|): the FP16 form meets
+    the 1e-5 bar against the restated reference too."""
+    x = np.random.default_rng(9).uniform(-1, 1, 1 << 16).astype(np.float32)
+    for exp in EXPS:
+        y = run(runner, x, ostft.parse_options("--hanning_window --remove_dc --mfma_fp16", exp, 0.5))
+        rc, r = po.pipeline(x, exp, 0.5, "--hanning_window --remove_dc")
+        assert rc == 0 and y.shape == r.shape
+        mx, _, zabs = frame_errors(y, r, 1 << exp)
+        assert mx <= 1e-5 and zabs <= 1e-6, (exp, mx)
+
+
+@pytest.mark.parametrize("flags", [0, N.FLAG_MFMA_FP16])
+def test_strided_batch_matches_single_requests(runner, flags):
+    import torch
+
+    rng = np.random.default_rng(4)
+    for exp in (6, 8, 10, 13):
+        if flags and exp == 13:
+            continue
+        count, L = 37, 20000 + exp
+        x = torch.from_numpy(rng.uniform(-1, 1, (count, L + 5)).astype(np.float32)).cuda()[:, 3: 3 + L]
+        p = ostft.make_params(exp, 0.5, N.WIN_HANNING, 0.0, N.OUT_COMPAT_POWER, N.FLAG_REMOVE_DC | flags)
+        out = runner.run_strided_device(x, p)
+        torch.cuda.synchronize()
+        for y in (0, 1, 17, count - 1):
+            one = runner.run_device(x[y].contiguous(), p)
+            torch.cuda.synchronize()
+            assert np.array_equal(out[y].cpu().numpy().view(np.uint32), one.cpu().numpy().view(np.uint32))
