@@ -157,15 +157,19 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
     // instruction of a wave covers a contiguous 1 KB
     const int mode = a.out_mode;
     const uint32_t BPT = mode == OUT_HALF_COMPLEX ? 2u : 4u;
+    // (the N/2+1-bin layouts take two consecutive bins of the flattened
+    // [frame][bin] index per thread and iteration)
     constexpr int OITER4 = (G * M) / (WG * 4);
     constexpr int OITER2 = 2 * OITER4;
+    constexpr int OITERP = (G * (M + 1) + 2 * WG - 1) / (2 * WG);
+    constexpr int NEB = OITERP > OITER2 ? OITERP : OITER2;
     const int oiter = mode == OUT_HALF_COMPLEX ? OITER2 : OITER4;
     const float2* __restrict__ twn = a.tw_n;
     const bool vec_out = mode == OUT_COMPAT_POWER || mode == OUT_MAGNITUDE || mode == OUT_HALF_COMPLEX;
-    float2 eb[OITER2];  // W_N^k0 for each epilogue iteration
+    float2 eb[NEB];  // W_N^k0 for each epilogue iteration
     float2 ec1 = make_float2(1.0f, 0.0f), ec2 = ec1, ec3 = ec1;
 #pragma unroll
-    for (int it = 0; it < OITER2; ++it)
+    for (int it = 0; it < NEB; ++it)
         eb[it] = ec1;
     if (vec_out && a.combine) {
 #pragma unroll
@@ -175,6 +179,16 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
         ec1 = twn[1];
         ec2 = twn[2];
         ec3 = twn[3];
+    } else if (a.combine) {
+#pragma unroll
+        for (int it = 0; it < OITERP; ++it) {
+            const uint32_t e1 = 2u * (tid + it * WG);
+            if (e1 < (uint32_t)(G * (M + 1))) {
+                const uint32_t k1 = e1 % (uint32_t)(M + 1);
+                eb[it] = twn[k1 == (uint32_t)M ? 0u : k1];
+            }
+        }
+        ec1 = twn[1];
     }
 
 #if OSTFT_SETPRIO
@@ -292,26 +306,59 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
             nt_store4(hi, o + (M + k0) / 4);
         }
     } else {
-        // POWER / HALF_SPECTRUM: M+1 values per frame, one value per thread
+        // POWER / HALF_SPECTRUM: M+1 values per frame; two consecutive values
+        // of the flattened [frame][bin] index per thread and iteration, one
+        // 8-byte (POWER) or 16-byte (HALF_SPECTRUM) store where aligned
         const uint32_t nval = nf * (uint32_t)(M + 1);
-        for (uint32_t e = tid; e < nval; e += WG) {
-            const uint32_t g = e / (M + 1), k = e - g * (M + 1);
-            const float2* z = lds + g * PADM;
-            const uint32_t kb = k == (uint32_t)M ? 0u : k;
-            const uint32_t km = (M - kb) & (M - 1);
-            const float2 w = a.combine ? twn[kb] : make_float2(1.0f, 0.0f);
-            const BinPair b = finish_bin(z[pad(kb)], z[pad(km)], w, a.combine, false);
-            const float2 xv = k == (uint32_t)M ? b.hi : b.lo;
-            const uint64_t o = (f0 * (uint64_t)(M + 1)) + e;
-            if (mode == OUT_POWER)
-                nt_store(xv.x * xv.x + xv.y * xv.y, out + o);
-            else
-                nt_store(xv, reinterpret_cast<float2*>(out) + o);
+        const uint64_t obase = f0 * (uint64_t)(M + 1);
+        const float2 one = make_float2(1.0f, 0.0f);
+#pragma unroll
+        for (int it = 0; it < OITERP; ++it) {
+            const uint32_t e1 = 2u * (tid + it * WG);
+            if (e1 >= nval)
+                break;
+            const bool has2 = e1 + 1 < nval;
+            const uint32_t g1 = e1 / (uint32_t)(M + 1), k1 = e1 - g1 * (uint32_t)(M + 1);
+            const uint32_t kb1 = k1 == (uint32_t)M ? 0u : k1;
+            const float2 w1 = a.combine ? eb[it] : one;
+            const float2* z1 = lds + g1 * PADM;
+            const BinPair b1 = finish_bin(z1[pad(kb1)], z1[pad((M - kb1) & (M - 1))], w1, a.combine, false);
+            const float2 x1 = k1 == (uint32_t)M ? b1.hi : b1.lo;
+            // the next value: bin k1 + 1 of the same frame, or bin 0 of the next
+            const uint32_t g2 = k1 == (uint32_t)M ? g1 + 1 : g1, k2 = k1 == (uint32_t)M ? 0u : k1 + 1;
+            const uint32_t kb2 = k2 == (uint32_t)M ? 0u : k2;
+            float2 x2 = make_float2(0.0f, 0.0f);
+            if (has2) {
+                const float2 w2 = (a.combine && kb2 != 0u) ? cmul(w1, ec1) : one;
+                const float2* z2 = lds + g2 * PADM;
+                const BinPair b2 = finish_bin(z2[pad(kb2)], z2[pad((M - kb2) & (M - 1))], w2, a.combine, false);
+                x2 = k2 == (uint32_t)M ? b2.hi : b2.lo;
+            }
+            const uint64_t o = obase + e1;
+            const bool pair = has2 && (o & 1u) == 0;
+            if (mode == OUT_POWER) {
+                const float p1 = x1.x * x1.x + x1.y * x1.y, p2 = x2.x * x2.x + x2.y * x2.y;
+                if (pair) {
+                    nt_store(make_float2(p1, p2), reinterpret_cast<float2*>(out + o));
+                } else {
+                    nt_store(p1, out + o);
+                    if (has2)
+                        nt_store(p2, out + o + 1);
+                }
+            } else {
+                float2* o2 = reinterpret_cast<float2*>(out) + o;
+                if (pair) {
+                    nt_store4(make_float4(x1.x, x1.y, x2.x, x2.y), reinterpret_cast<float4*>(o2));
+                } else {
+                    nt_store(x1, o2);
+                    if (has2)
+                        nt_store(x2, o2 + 1);
+                }
+            }
         }
     }
 }
 
-// One request, or a strided batch of equal requests (signal y = blockIdx.y).
 template <int LOG_M, int XF>
 __global__ void __launch_bounds__(Cfg<LOG_M>::WG) stft_fused_kernel(const FusedArgs a)
 {

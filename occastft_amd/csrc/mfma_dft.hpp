@@ -139,9 +139,10 @@ __device__ __forceinline__ void mfma_stage(float2* lds, const float2* __restrict
 
     // gather V for every column block of this wave (all reads before the
     // in-place writes)
+    // compile-time loops over the column blocks: u[] must stay in registers
     float2 u[GPW][4];
-#pragma unroll
-    for (int gi = 0; gi < GPW; ++gi) {
+    cfor<0, GPW>([&](auto gic) {
+        constexpr int gi = decltype(gic)::value;
         const uint32_t col = (wave + gi * WAVES) * 16u + i;
         const uint32_t f = col / NB, b = col % NB;
 #pragma unroll
@@ -152,11 +153,11 @@ __device__ __forceinline__ void mfma_stage(float2* lds, const float2* __restrict
                 x = cmul(x, tw[((b % S) * m) * (M / (S * R))]);
             u[gi][j] = x;
         }
-    }
+    });
     __syncthreads();
 
-#pragma unroll
-    for (int gi = 0; gi < GPW; ++gi) {
+    cfor<0, GPW>([&](auto gic) {
+        constexpr int gi = decltype(gic)::value;
         const uint32_t col = (wave + gi * WAVES) * 16u + i;
         const uint32_t f = col / NB, b = col % NB;
         // the column's scale: its largest component over the 4 lanes holding it
@@ -213,7 +214,7 @@ __device__ __forceinline__ void mfma_stage(float2* lds, const float2* __restrict
                 }
             }
         }
-    }
+    });
     __syncthreads();
 }
 
