@@ -1,4 +1,4 @@
-// multipass.hpp -- large-window path (complex transform sizes 2^14 .. 2^20).
+// multipass.hpp -- large-window path (complex transform sizes 2^13 .. 2^19).
 #pragma once
 
 #include "engine.hpp"
@@ -20,10 +20,10 @@ struct Scratch {
 float2* ctx_twiddles(ostft_ctx* ctx, int64_t m, int64_t count, int* rc);
 Scratch& ctx_scratch(ostft_ctx* ctx);
 
-// Runs the P column passes and the split epilogue.  Returns an OSTFT_* status.
+// Runs the two passes (the split epilogue inside the last).  Returns an OSTFT_* status.
 int run_multipass(ostft_ctx* ctx, int log_m, const FusedArgs& a, hipStream_t s);
 
-// Radix split of 2^log_m into passes of at most 2^8 (larger radix first).
+// Radix split of 2^log_m into the two passes (returns 2; 0 outside 13..19).
 int multipass_plan(int log_m, int* log_radix /* [4] */);
 
 }  // namespace ostft
