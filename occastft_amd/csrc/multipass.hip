@@ -11,10 +11,10 @@
 // LDS): CB = 8192/R butterflies of one frame, gathered as R rows of CB
 // contiguous values (stride M/R), FFT'd as CB independent R-point transforms
 // with the fused kernel's LDS Stockham engine.
-//   * first pass (radix R0 <= 2^10): reads the signal itself (overlap gather +
+//   * first pass (radix R0 <= 2^11): reads the signal itself (overlap gather +
 //     window), writes the Stockham outputs Y[g R0 + q] (each butterfly's R0
 //     results contiguous);
-//   * last pass (radix R1 <= 2^9, stride Ns = R0): applies the inter-pass
+//   * last pass (radix R1 <= 2^8, stride Ns = R0): applies the inter-pass
 //     twiddle W_M^{g m} from a two-level FP64 table, and holds a block of
 //     butterflies g together with its mirror block Ns - g (0 and Ns/2 are their
 //     own mirrors), so both Z[k] and Z[M-k] of every output index k are in its
@@ -223,11 +223,12 @@ __global__ void __launch_bounds__(MP_THREADS) last_pass_kernel(const LastArgs a)
 
 int multipass_plan(int log_m, int* lr)
 {
-    // two passes: the last radix at most 2^9 (a block and its mirror block
-    // share one workgroup), the first at most 2^10
+    // two passes: the last radix at most 2^8 (a block and its mirror block of
+    // at least 16 butterflies share one workgroup, so output runs are >= 64 B;
+    // radix 2^9 measured 1.7x slower), the first at most 2^11
     if (log_m < 13 || log_m > 19)
         return 0;
-    lr[1] = log_m / 2 > 9 ? 9 : log_m / 2;
+    lr[1] = log_m / 2 > 8 ? 8 : log_m / 2;
     lr[0] = log_m - lr[1];
     return 2;
 }
@@ -291,6 +292,7 @@ int run_multipass(ostft_ctx* ctx, int log_m, const FusedArgs& a, hipStream_t s)
     case 8: e = launch_first<8>(p, s); break;
     case 9: e = launch_first<9>(p, s); break;
     case 10: e = launch_first<10>(p, s); break;
+    case 11: e = launch_first<11>(p, s); break;
     default: return OSTFT_ERR_UNSUPPORTED;
     }
     if (e != hipSuccess)
@@ -314,7 +316,6 @@ int run_multipass(ostft_ctx* ctx, int log_m, const FusedArgs& a, hipStream_t s)
     case 6: e = launch_last<6>(la, s); break;
     case 7: e = launch_last<7>(la, s); break;
     case 8: e = launch_last<8>(la, s); break;
-    case 9: e = launch_last<9>(la, s); break;
     default: return OSTFT_ERR_UNSUPPORTED;
     }
     return e == hipSuccess ? OSTFT_OK : OSTFT_ERR_HIP;
