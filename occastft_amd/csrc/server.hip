@@ -29,6 +29,7 @@
 #include <mutex>
 #include <string>
 #include <thread>
+#include <tuple>
 #include <unordered_map>
 #include <vector>
 
@@ -90,6 +91,83 @@ struct Ring {
     void pop() { chunks.pop_front(); }
 };
 
+// A few workers for the host copies between callers' buffers and the pinned
+// mirror: one thread moves ~10 GB/s, less than the PCIe link carries.
+class CopyPool {
+public:
+    explicit CopyPool(int n)
+    {
+        for (int i = 0; i < n; ++i)
+            workers_.emplace_back([this] { loop(); });
+    }
+    ~CopyPool()
+    {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (auto& t : workers_)
+            t.join();
+    }
+    // Copies every (dst, src, bytes) job, split into pieces of at most 4 MiB,
+    // with the calling thread helping; returns when all are done.
+    void copy(const std::vector<std::tuple<void*, const void*, size_t>>& jobs)
+    {
+        constexpr size_t PIECE = 4u << 20;
+        std::vector<std::tuple<char*, const char*, size_t>> pieces;
+        for (const auto& j : jobs) {
+            char* d = static_cast<char*>(std::get<0>(j));
+            const char* sp = static_cast<const char*>(std::get<1>(j));
+            for (size_t o = 0; o < std::get<2>(j); o += PIECE)
+                pieces.emplace_back(d + o, sp + o, std::min(PIECE, std::get<2>(j) - o));
+        }
+        if (pieces.empty())
+            return;
+        std::unique_lock<std::mutex> lk(mu_);
+        pieces_ = &pieces;
+        next_ = 0;
+        left_ = pieces.size();
+        ++gen_;
+        cv_.notify_all();
+        work(lk);
+        done_.wait(lk, [&] { return left_ == 0; });
+        pieces_ = nullptr;
+    }
+
+private:
+    void work(std::unique_lock<std::mutex>& lk)
+    {
+        while (pieces_ && next_ < pieces_->size()) {
+            const auto pc = (*pieces_)[next_++];
+            lk.unlock();
+            std::memcpy(std::get<0>(pc), std::get<1>(pc), std::get<2>(pc));
+            lk.lock();
+            if (--left_ == 0)
+                done_.notify_all();
+        }
+    }
+    void loop()
+    {
+        std::unique_lock<std::mutex> lk(mu_);
+        uint64_t seen = 0;
+        for (;;) {
+            cv_.wait(lk, [&] { return stop_ || (gen_ != seen && pieces_ && next_ < pieces_->size()); });
+            if (stop_)
+                return;
+            seen = gen_;
+            work(lk);
+        }
+    }
+    std::mutex mu_;
+    std::condition_variable cv_, done_;
+    std::vector<std::thread> workers_;
+    const std::vector<std::tuple<char*, const char*, size_t>>* pieces_ = nullptr;
+    size_t next_ = 0, left_ = 0;
+    uint64_t gen_ = 0;
+    bool stop_ = false;
+};
+
 }  // namespace
 
 struct ostft_server {
@@ -117,6 +195,7 @@ struct ostft_server {
     // statistics
     ostft_server_stats st{};
     std::vector<double> lat_ms;
+    std::unique_ptr<CopyPool> copy_in, copy_out;  // one pool per direction (two threads use them)
     Clock::time_point t_first, t_last;
     bool have_first = false;
 
@@ -158,9 +237,13 @@ int ostft_server::launch(Batch* b)
     hipError_t e = hipSetDevice(device);
     if (e != hipSuccess)
         return OSTFT_ERR_NO_DEVICE;
-    for (Request* r : b->reqs)
-        if (r->L)
-            std::memcpy(arena_h + r->in_off, r->in, sizeof(float) * r->L);
+    {
+        std::vector<std::tuple<void*, const void*, size_t>> jobs;
+        for (Request* r : b->reqs)
+            if (r->L)
+                jobs.emplace_back(arena_h + r->in_off, r->in, sizeof(float) * r->L);
+        copy_in->copy(jobs);
+    }
     if (b->in_bytes)
         e = hipMemcpyAsync(arena_d + b->start, arena_h + b->start, b->in_bytes, hipMemcpyHostToDevice, h2d);
     if (e == hipSuccess)
@@ -295,10 +378,13 @@ void ostft_server::completion_loop()
                 b->err = hipGetErrorString(e);
             }
         }
-        if (status == OSTFT_OK)
+        if (status == OSTFT_OK) {
+            std::vector<std::tuple<void*, const void*, size_t>> jobs;
             for (Request* r : b->reqs)
                 if (r->g.out_floats)
-                    std::memcpy(r->out, arena_h + r->out_off, sizeof(float) * r->g.out_floats);
+                    jobs.emplace_back(r->out, arena_h + r->out_off, sizeof(float) * r->g.out_floats);
+            copy_out->copy(jobs);
+        }
         const Clock::time_point now = Clock::now();
         lk.lock();
         for (Request* r : b->reqs)
@@ -377,6 +463,8 @@ int ostft_server_create(int32_t device, const ostft_server_config* config, ostft
         return e != hipSuccess ? OSTFT_ERR_HIP : rc;
     }
     s->ring.cap = cfg.arena_bytes;
+    s->copy_in.reset(new CopyPool(3));
+    s->copy_out.reset(new CopyPool(3));
     ostft_server* raw = s.get();
     raw->sched = std::thread([raw] { raw->scheduler_loop(); });
     raw->comp = std::thread([raw] { raw->completion_loop(); });
