@@ -717,7 +717,7 @@ def main() -> int:
     cold_ms = float(np.mean(cold))
     achieved = b / (cold_ms / 1e3) / 1e9
     total_bytes = sum(algorithmic_bytes(g, 1 << e) for e, ov, name, p, g in reqs)
-    kernel_key = f"stft_fused_kernel<{e - 1}> window {1 << e} overlap {ov} {name}"
+    kernel_key = f"stft_fused_kernel<{e - 1}, 0> window {1 << e} overlap {ov} {name}"
     traffic = measured_traffic(kernel_key)
 
     result = {

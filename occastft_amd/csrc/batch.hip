@@ -359,7 +359,10 @@ int run_batch(ostft_ctx* ctx, const ostft_job* jobs, uint32_t njobs, hipStream_t
         e = hipMemcpyAsync(sl.d, sl.h, off, hipMemcpyHostToDevice, s);
         if (e != hipSuccess)
             return ctx_hip_fail(ctx, e, "batch descriptor upload");
-        if (st->use_graphs) {
+        // the captured sequence holds a node per window size (an absent size
+        // still costs one exiting workgroup, ~3.5 us): replay it when at least
+        // three sizes are present, launch directly otherwise
+        if (st->use_graphs && __builtin_popcount(mask) >= 3) {
             e = replay_slot(st, si, mask, ntiles, s);
         } else {
             e = launch_classes(st, mask, sl.d, s, ntiles);
