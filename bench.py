@@ -624,6 +624,40 @@ def bench_serve(args) -> dict:
     return res
 
 
+def launch_only(args) -> int:
+    """Counter-collection runs (no timing, no JSON): three launches of each
+    `exp:overlap:layout` of --launch on the 1e8-sample sweep signal
+    (layout power | hc), or with --launch-strided two strided launches per
+    window 64/256/1024 of 512 x 2^18 Gaussian samples, FP32 and MFMA FP16,
+    complex half spectrum."""
+    import torch
+
+    import occastft_amd as ostft
+    from occastft_amd import _native as N
+
+    r = ostft.Runner(0)
+    if args.launch_strided:
+        x = torch.randn(512, 1 << 18, device="cuda", generator=torch.Generator(device="cuda").manual_seed(7))
+        for e in (6, 8, 10):
+            for flags in (0, N.FLAG_MFMA_FP16):
+                p = ostft.make_params(e, 0.5, N.WIN_HANNING, 0.0, N.OUT_HALF_SPECTRUM, flags)
+                for _ in range(2):
+                    r.run_strided_device(x, p)
+    else:
+        x = make_signal(SIGNAL_LEN, SEED)
+        out = torch.empty(2 * SIGNAL_LEN + 65536, dtype=torch.float32, device="cuda")
+        for spec in args.launch:
+            e, ov, lay = spec.split(":")
+            opts = "--hanning_window --remove_dc" + (" --half_complex_return" if lay == "hc" else "")
+            p = ostft.parse_options(opts, int(e), float(ov))
+            for _ in range(3):
+                r.run_device(x, p, out=out)
+    torch.cuda.synchronize()
+    r.close()
+    print("launches done", flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -635,6 +669,10 @@ def main() -> int:
                     help="only time a plain 1 GiB device copy (achievable HBM bandwidth; counter calibration)")
     ap.add_argument("--workload", choices=("sweep", "large", "batch", "serve"), default="sweep")
     ap.add_argument("--rate", type=float, default=1000.0, help="serve: Poisson arrival rate per second")
+    ap.add_argument("--launch", nargs="+", metavar="EXP:OVERLAP:LAYOUT",
+                    help="counter runs: only launch these sweep configs (3 times each)")
+    ap.add_argument("--launch-strided", action="store_true",
+                    help="counter runs: only launch the strided FP32 / MFMA batches")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: frames are independent, so N GPUs would each run this "
@@ -648,6 +686,8 @@ def main() -> int:
     torch.cuda.set_device(0)
     if args.copy_probe:
         return copy_probe(args)
+    if args.launch or args.launch_strided:
+        return launch_only(args)
     if args.workload != "sweep":
         fn = {"large": bench_large, "batch": bench_batch, "serve": bench_serve}[args.workload]
         print(json.dumps(fn(args)))

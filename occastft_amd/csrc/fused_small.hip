@@ -38,6 +38,12 @@ namespace ostft {
 #ifndef OSTFT_PAIR_LDS
 #define OSTFT_PAIR_LDS 1
 #endif
+// complex sizes from 2^OSTFT_DIRECT_LOG_M on load each frame's samples straight
+// into registers (8-byte loads per thread, the overlap halo from L2) instead of
+// staging the workgroup's span in LDS
+#ifndef OSTFT_DIRECT_LOG_M
+#define OSTFT_DIRECT_LOG_M 99
+#endif
 // Minimum waves per SIMD the persistent batch kernel is compiled for: 4 (a
 // 128-register budget, which LDS allows anyway) where that compiles without
 // spills, else 1 (the compiler's choice).
@@ -117,27 +123,48 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
     // (b) the window coefficients of this thread's 16 packed samples;
     // (c) the base twiddles of every Stockham pass;
     // (d) W_N^k for the epilogue bins of this thread.
+    constexpr bool DIRECT = LOG_M >= OSTFT_DIRECT_LOG_M;
     const uint64_t s_begin = f0 * hop;
     const uint64_t s_al = s_begin & ~3ull;
     const uint64_t s_end = (f0 + nf - 1) * hop + N;
     const uint32_t nvec = (uint32_t)((s_end - s_al + 3) >> 2);
     const bool vec_ok = (reinterpret_cast<uintptr_t>(in) & 15u) == 0;
     constexpr int MAXVEC = G * N / 4 + 2;
-    constexpr int ITER = (MAXVEC + WG - 1) / WG;
+    constexpr int ITER = DIRECT ? 1 : (MAXVEC + WG - 1) / WG;
     float4 x[ITER];
+    float2 xd[DIRECT ? 16 : 1];
+    if constexpr (DIRECT) {
+        // the thread's packed samples (x[2n], x[2n+1]), n = t + m T
+        const uint64_t s0 = (f0 + grp) * hop;
+        const bool pair_ok = ((reinterpret_cast<uintptr_t>(in + s0)) & 7u) == 0;
 #pragma unroll
-    for (int it = 0; it < ITER; ++it) {
-        const uint32_t i = tid + it * WG;
-        const uint64_t s = s_al + 4ull * i;
-        x[it] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        if (i < nvec) {
-            if (vec_ok && s + 4 <= L) {
-                x[it] = *reinterpret_cast<const float4*>(in + s);
-            } else {
-                x[it].x = s < L ? in[s] : 0.0f;
-                x[it].y = s + 1 < L ? in[s + 1] : 0.0f;
-                x[it].z = s + 2 < L ? in[s + 2] : 0.0f;
-                x[it].w = s + 3 < L ? in[s + 3] : 0.0f;
+        for (int m = 0; m < 16; ++m) {
+            const uint64_t s = s0 + 2ull * (t + m * T);
+            xd[m] = make_float2(0.0f, 0.0f);
+            if (active) {
+                if (pair_ok && s + 2 <= L) {
+                    xd[m] = *reinterpret_cast<const float2*>(in + s);
+                } else {
+                    xd[m].x = s < L ? in[s] : 0.0f;
+                    xd[m].y = s + 1 < L ? in[s + 1] : 0.0f;
+                }
+            }
+        }
+    } else {
+#pragma unroll
+        for (int it = 0; it < ITER; ++it) {
+            const uint32_t i = tid + it * WG;
+            const uint64_t s = s_al + 4ull * i;
+            x[it] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (i < nvec) {
+                if (vec_ok && s + 4 <= L) {
+                    x[it] = *reinterpret_cast<const float4*>(in + s);
+                } else {
+                    x[it].x = s < L ? in[s] : 0.0f;
+                    x[it].y = s + 1 < L ? in[s + 1] : 0.0f;
+                    x[it].z = s + 2 < L ? in[s + 2] : 0.0f;
+                    x[it].w = s + 3 < L ? in[s + 3] : 0.0f;
+                }
             }
         }
     }
@@ -194,19 +221,23 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
 #if OSTFT_SETPRIO
     __builtin_amdgcn_s_setprio(0);
 #endif
-    // stage the span in LDS
-#pragma unroll
-    for (int it = 0; it < ITER; ++it) {
-        const uint32_t i = tid + it * WG;
-        if (i < nvec)
-            *reinterpret_cast<float4*>(sf + spad(4 * i)) = x[it];
-    }
-    __syncthreads();
-
     // ---- 2. each group gathers its frame, packed z[n] = x[2n] + i x[2n+1],
     //         and applies the window (Window_*) ----
     float2 v[16];
-    {
+    if constexpr (DIRECT) {
+#pragma unroll
+        for (int m = 0; m < 16; ++m)
+            v[m] = w2 ? make_float2(xd[m].x * wv[m].x, xd[m].y * wv[m].y) : xd[m];
+    } else {
+        // stage the span in LDS
+#pragma unroll
+        for (int it = 0; it < ITER; ++it) {
+            const uint32_t i = tid + it * WG;
+            if (i < nvec)
+                *reinterpret_cast<float4*>(sf + spad(4 * i)) = x[it];
+        }
+        __syncthreads();
+
         const uint32_t q0 = (uint32_t)(grp * hop + (s_begin - s_al));
         // an even q0 keeps every (x[2n], x[2n+1]) pair inside one padded row
         const bool pair_ok = (q0 & 1u) == 0;
@@ -225,8 +256,8 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
                 xv = make_float2(xv.x * wv[m].x, xv.y * wv[m].y);
             v[m] = xv;
         }
+        __syncthreads();  // the FFT reuses the staging buffer
     }
-    __syncthreads();  // the FFT reuses the staging buffer
 
     // ---- 3. DC removal of the windowed frame (DCRemove_Common) ----
     if (a.dc) {
