@@ -16,9 +16,12 @@
 
 namespace ostft {
 
+// Twice the outputs at positions k and M + k: the halvings of E and O are
+// left to the output conversion (one exact scaling by a power of two there,
+// folded into a constant for the real layouts), which saves vector ALU work.
 struct BinPair {
-    float2 lo;  // position k
-    float2 hi;  // position M + k
+    float2 lo;  // 2 X[k]
+    float2 hi;  // 2 X[M + k]
 };
 
 // combine != 0: final radix-2 stage with w = W_N^k; otherwise [E, O].
@@ -26,8 +29,8 @@ struct BinPair {
 // which moves (E0, O0) to ((E0-O0)/2, (O0-E0)/2).
 __device__ __forceinline__ BinPair finish_bin(float2 zk, float2 zm, float2 w, int combine, bool dc0)
 {
-    float2 E = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
-    float2 O = make_float2(0.5f * (zk.y + zm.y), 0.5f * (zm.x - zk.x));
+    float2 E = make_float2(zk.x + zm.x, zk.y - zm.y);  // 2 E
+    float2 O = make_float2(zk.y + zm.y, zm.x - zk.x);  // 2 O
     if (dc0) {
         const float d = 0.5f * (E.x - O.x);
         E = make_float2(d, 0.0f);
@@ -45,12 +48,23 @@ __device__ __forceinline__ BinPair finish_bin(float2 zk, float2 zm, float2 w, in
     return r;
 }
 
+// the output conversions of a doubled value x2 = 2 X:
 // sqrt(Re^2 + Re^2) of the Serial runner's ToPower, as sqrt(2)|Re|
-__device__ __forceinline__ float compat_power(float2 x) { return 1.41421356237309504880f * fabsf(x.x); }
-// |x| with the hardware square root (1 ulp)
-__device__ __forceinline__ float magnitude(float2 x) { return __builtin_amdgcn_sqrtf(x.x * x.x + x.y * x.y); }
+__device__ __forceinline__ float compat_power(float2 x2)
+{
+    constexpr float HALF_SQRT2 = 1.41421356237309504880f * 0.5f;  // exactly half the float sqrt(2)
+    return HALF_SQRT2 * fabsf(x2.x);
+}
+// |X| with the hardware square root (1 ulp)
+__device__ __forceinline__ float magnitude(float2 x2)
+{
+    return 0.5f * __builtin_amdgcn_sqrtf(x2.x * x2.x + x2.y * x2.y);
+}
+__device__ __forceinline__ float power_of(float2 x2) { return 0.25f * (x2.x * x2.x + x2.y * x2.y); }
+__device__ __forceinline__ float2 value_of(float2 x2) { return make_float2(0.5f * x2.x, 0.5f * x2.y); }
 
-// Write the outputs of bin k of frame f (frame length N = 2M).
+// Write the outputs of bin k of frame f (frame length N = 2M) from the
+// doubled pair b.
 //   COMPAT_POWER  sqrt(Re^2 + Re^2) at f*N + k and f*N + M + k
 //   MAGNITUDE     |.| at the same places
 //   HALF_COMPLEX  (Re, Im) at complex index f*N + k, f*N + M + k, below `limit`
@@ -72,23 +86,23 @@ __device__ __forceinline__ void write_bin(float* __restrict__ out, int mode, uin
     case OUT_HALF_COMPLEX: {
         float2* o2 = reinterpret_cast<float2*>(out);
         if (base + k < limit)
-            nt_store(b.lo, o2 + base + k);
+            nt_store(value_of(b.lo), o2 + base + k);
         if (base + M + k < limit)
-            nt_store(b.hi, o2 + base + M + k);
+            nt_store(value_of(b.hi), o2 + base + M + k);
         break;
     }
     case OUT_POWER: {
         float* o = out + f * (uint64_t)(M + 1);
-        nt_store(b.lo.x * b.lo.x + b.lo.y * b.lo.y, o + k);
+        nt_store(power_of(b.lo), o + k);
         if (k == 0)
-            nt_store(b.hi.x * b.hi.x + b.hi.y * b.hi.y, o + M);
+            nt_store(power_of(b.hi), o + M);
         break;
     }
     case OUT_HALF_SPECTRUM: {
         float2* o = reinterpret_cast<float2*>(out) + f * (uint64_t)(M + 1);
-        nt_store(b.lo, o + k);
+        nt_store(value_of(b.lo), o + k);
         if (k == 0)
-            nt_store(b.hi, o + M);
+            nt_store(value_of(b.hi), o + M);
         break;
     }
     default:

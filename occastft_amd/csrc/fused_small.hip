@@ -150,6 +150,15 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
                 }
             }
         }
+    } else if (vec_ok && s_al + 4ull * nvec <= L) {
+        // the whole span is inside the signal: 16-byte loads, no per-load
+        // checks (values past nvec are never staged)
+#pragma unroll
+        for (int it = 0; it < ITER; ++it) {
+            const uint32_t i = tid + it * WG;
+            if (i < nvec)
+                x[it] = *reinterpret_cast<const float4*>(in + s_al + 4ull * i);
+        }
     } else {
 #pragma unroll
         for (int it = 0; it < ITER; ++it) {
@@ -301,9 +310,11 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
             const BinPair b1 = finish_bin(z[pad(k0 + 1)], z[pad(M - k0 - 1)], cmul(eb[it], ec1), a.combine, false);
             const uint64_t base = (f0 + g) * (uint64_t)N;
             if (base + k0 < lim)
-                nt_store4(make_float4(b0.lo.x, b0.lo.y, b1.lo.x, b1.lo.y), o + (base + k0) / 2);
+                nt_store4(make_float4(0.5f * b0.lo.x, 0.5f * b0.lo.y, 0.5f * b1.lo.x, 0.5f * b1.lo.y),
+                          o + (base + k0) / 2);
             if (base + M + k0 < lim)
-                nt_store4(make_float4(b0.hi.x, b0.hi.y, b1.hi.x, b1.hi.y), o + (base + M + k0) / 2);
+                nt_store4(make_float4(0.5f * b0.hi.x, 0.5f * b0.hi.y, 0.5f * b1.hi.x, 0.5f * b1.hi.y),
+                          o + (base + M + k0) / 2);
         }
     } else if (vec_out) {
         const uint32_t nbins = nf * (uint32_t)M;
@@ -354,7 +365,7 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
             const float2 w1 = a.combine ? eb[it] : one;
             const float2* z1 = lds + g1 * PADM;
             const BinPair b1 = finish_bin(z1[pad(kb1)], z1[pad((M - kb1) & (M - 1))], w1, a.combine, false);
-            const float2 x1 = k1 == (uint32_t)M ? b1.hi : b1.lo;
+            const float2 x1 = k1 == (uint32_t)M ? b1.hi : b1.lo;  // doubled
             // the next value: bin k1 + 1 of the same frame, or bin 0 of the next
             const uint32_t g2 = k1 == (uint32_t)M ? g1 + 1 : g1, k2 = k1 == (uint32_t)M ? 0u : k1 + 1;
             const uint32_t kb2 = k2 == (uint32_t)M ? 0u : k2;
@@ -368,7 +379,7 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
             const uint64_t o = obase + e1;
             const bool pair = has2 && (o & 1u) == 0;
             if (mode == OUT_POWER) {
-                const float p1 = x1.x * x1.x + x1.y * x1.y, p2 = x2.x * x2.x + x2.y * x2.y;
+                const float p1 = power_of(x1), p2 = power_of(x2);
                 if (pair) {
                     nt_store(make_float2(p1, p2), reinterpret_cast<float2*>(out + o));
                 } else {
@@ -378,12 +389,13 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
                 }
             } else {
                 float2* o2 = reinterpret_cast<float2*>(out) + o;
+                const float2 v1 = value_of(x1), v2 = value_of(x2);
                 if (pair) {
-                    nt_store4(make_float4(x1.x, x1.y, x2.x, x2.y), reinterpret_cast<float4*>(o2));
+                    nt_store4(make_float4(v1.x, v1.y, v2.x, v2.y), reinterpret_cast<float4*>(o2));
                 } else {
-                    nt_store(x1, o2);
+                    nt_store(v1, o2);
                     if (has2)
-                        nt_store(x2, o2 + 1);
+                        nt_store(v2, o2 + 1);
                 }
             }
         }
