@@ -245,6 +245,16 @@ int ostft_server_wait(ostft_server* srv, uint64_t ticket, int32_t timeout_ms, ui
 int ostft_server_stats_get(ostft_server* srv, ostft_server_stats* out, int32_t reset);
 const char* ostft_server_last_error(const ostft_server* srv);
 
+/* ostft_activate with the frame geometry given instead of derived from the
+ * overlap ratio: `hop` (OMove) and `frames` (qtConst = OFullSize / N), as the
+ * reference's per-stage functions receive them (runnerFunction::Overlap and
+ * ToPower / HalfComplex, StandAlone/cross_gpgpu/GLOBAL/RunnerInterface.hpp:38-83).
+ * params->overlap is not used.  Used by the record-and-fuse implementation of
+ * those functions (occastft_runner_functions.hpp). */
+int ostft_activate_geometry(ostft_ctx* ctx, const ostft_params* params, const float* in,
+                            uint64_t full_size, uint64_t hop, uint64_t frames, float* out,
+                            uint64_t out_cap, uint64_t* out_len);
+
 /* Host-side window coefficients the engine uses by default for `kind`, size n:
  * the reference's own float/double evaluation (OKL/kernel.okl:33-127 as
  * translated in StandAlone/cross_gpgpu/Serial/kernel/compiled.hpp:20-88).

@@ -52,6 +52,7 @@ EXPORTS = (
     "ostft_window_coefficients",
     "ostft_run_batch_device",
     "ostft_run_strided_device",
+    "ostft_activate_geometry",
     "ostft_batch_config",
     "ostft_autotune_batch",
     "ostft_server_create",
@@ -179,6 +180,9 @@ def lib() -> ctypes.CDLL:
     L.ostft_run_strided_device.restype = ctypes.c_int
     L.ostft_run_strided_device.argtypes = [vp, ctypes.POINTER(Params), vp, ctypes.c_uint64, ctypes.c_uint64,
                                            ctypes.c_uint32, vp, ctypes.c_uint64, vp]
+    L.ostft_activate_geometry.restype = ctypes.c_int
+    L.ostft_activate_geometry.argtypes = [vp, ctypes.POINTER(Params), fp, ctypes.c_uint64, ctypes.c_uint64,
+                                          ctypes.c_uint64, fp, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)]
     L.ostft_batch_config.restype = ctypes.c_int
     L.ostft_batch_config.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, i32p]
     L.ostft_autotune_batch.restype = ctypes.c_int

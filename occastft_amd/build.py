@@ -90,7 +90,35 @@ def build_runner(verbose: bool = False) -> str:
     return RUNNER
 
 
+PLUGIN_LIB = os.path.join(HERE, "libocca_stft_runnerfn.so")
+PLUGIN_DRIVER = os.path.join(HERE, "runnerfn_pipeline")
+
+
+def build_plugin(verbose: bool = False) -> str:
+    """Link the record-and-fuse runner functions (libocca_stft_runnerfn.so,
+    host C++) and the pipeline driver that exercises them."""
+    cxx = shutil.which("g++") or "g++"
+    inc = ["-I", os.path.join(HERE, "..", "include")]
+    link = ["-L", HERE, "-locca_stft_hip", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath-link,/opt/rocm/lib"]
+    steps = [
+        (PLUGIN_LIB, [cxx, "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", *inc,
+                      os.path.join(CSRC, "plugin", "runner_functions.cpp"), *link]),
+        (PLUGIN_DRIVER, [cxx, "-O2", "-std=c++17", "-Wall", *inc, os.path.join(CSRC, "plugin", "pipeline_driver.cpp"),
+                         "-L", HERE, "-locca_stft_runnerfn", *link]),
+    ]
+    for out, cmd in steps:
+        tmp = out + ".tmp"
+        r = subprocess.run([*cmd, "-o", tmp], capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"build failed for {out}:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+        os.replace(tmp, out)
+        if verbose:
+            print(f"built {out}")
+    return PLUGIN_LIB
+
+
 if __name__ == "__main__":
     build(verbose=True)
     build_runner(verbose=True)
+    build_plugin(verbose=True)
     sys.exit(0)

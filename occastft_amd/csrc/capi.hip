@@ -106,7 +106,10 @@ struct Plan {
     int combine;  // 0: stop before the last radix-2 stage (odd-exponent result)
 };
 
-int make_plan(const ostft_params* p, uint64_t L, Plan* plan, ostft_ctx* ctx)
+// hop/frames != 0 replace the float32 geometry (the runnerFunction stage
+// functions receive OMove and OFullSize already computed by the caller).
+int make_plan(const ostft_params* p, uint64_t L, Plan* plan, ostft_ctx* ctx, uint64_t hop_in = 0,
+              uint64_t frames_in = 0)
 {
     if (!p)
         return fail(ctx, OSTFT_ERR_INVALID_ARG, "null params");
@@ -131,6 +134,12 @@ int make_plan(const ostft_params* p, uint64_t L, Plan* plan, ostft_ctx* ctx)
     g.window_size = N;
     g.frames = (uint64_t)q;
     g.hop = omove_f32((float)N, p->overlap);
+    if (hop_in || frames_in) {
+        if (!hop_in || !frames_in || frames_in > 2147483647ull)
+            return fail(ctx, OSTFT_ERR_INVALID_ARG, "explicit geometry needs hop >= 1 and 1 <= frames < 2^31");
+        g.hop = hop_in;
+        g.frames = frames_in;
+    }
     g.out_full = g.frames * N;
     g.out_half = g.out_full / 2;
     switch (p->output) {
@@ -273,12 +282,12 @@ int ctx_hip_fail(ostft_ctx* ctx, hipError_t e, const char* where) { return hip_f
 // Validates one device-resident request and resolves its tables (window,
 // twiddles): everything a launch needs, in `a`.  The caller's device is current.
 int ctx_prepare(ostft_ctx* ctx, const ostft_params* params, const float* d_in, uint64_t full_size, float* d_out,
-                uint64_t out_cap, FusedArgs* a, int* log_m)
+                uint64_t out_cap, FusedArgs* a, int* log_m, uint64_t hop, uint64_t frames)
 {
     if (!params || !d_out || (!d_in && full_size > 0))
         return fail(ctx, OSTFT_ERR_INVALID_ARG, "null argument");
     Plan plan;
-    int rc = make_plan(params, full_size, &plan, ctx);
+    int rc = make_plan(params, full_size, &plan, ctx, hop, frames);
     if (rc != OSTFT_OK)
         return rc;
     if (out_cap < plan.g.out_floats)
@@ -484,17 +493,15 @@ void ostft_destroy(ostft_ctx* ctx)
     delete ctx;
 }
 
-int ostft_run_device(ostft_ctx* ctx, const ostft_params* params, const float* d_in, uint64_t full_size,
-                     float* d_out, uint64_t out_cap, void* hip_stream)
+}  // extern "C"
+
+namespace {
+int run_device_geometry(ostft_ctx* ctx, const ostft_params* params, const float* d_in, uint64_t full_size,
+                        float* d_out, uint64_t out_cap, void* hip_stream, uint64_t hop, uint64_t frames)
 {
-    if (!ctx)
-        return OSTFT_ERR_INVALID_ARG;
-    DeviceGuard dg(ctx->device);
-    if (!dg.ok)
-        return fail(ctx, OSTFT_ERR_NO_DEVICE, "hipSetDevice failed");
     ostft::FusedArgs a;
     int log_m = 0;
-    const int rc = ostft::ctx_prepare(ctx, params, d_in, full_size, d_out, out_cap, &a, &log_m);
+    const int rc = ostft::ctx_prepare(ctx, params, d_in, full_size, d_out, out_cap, &a, &log_m, hop, frames);
     if (rc != OSTFT_OK)
         return rc;
     hipStream_t s = static_cast<hipStream_t>(hip_stream);
@@ -507,6 +514,20 @@ int ostft_run_device(ostft_ctx* ctx, const ostft_params* params, const float* d_
         return OSTFT_OK;
     }
     return ostft::run_multipass(ctx, log_m, a, s);
+}
+}  // namespace
+
+extern "C" {
+
+int ostft_run_device(ostft_ctx* ctx, const ostft_params* params, const float* d_in, uint64_t full_size,
+                     float* d_out, uint64_t out_cap, void* hip_stream)
+{
+    if (!ctx)
+        return OSTFT_ERR_INVALID_ARG;
+    DeviceGuard dg(ctx->device);
+    if (!dg.ok)
+        return fail(ctx, OSTFT_ERR_NO_DEVICE, "hipSetDevice failed");
+    return run_device_geometry(ctx, params, d_in, full_size, d_out, out_cap, hip_stream, 0, 0);
 }
 
 int ostft_run_strided_device(ostft_ctx* ctx, const ostft_params* params, const float* d_in, uint64_t full_size,
@@ -586,6 +607,11 @@ int ostft_autotune_batch(ostft_ctx* ctx, int32_t exp, uint64_t samples, int32_t*
     return ostft::autotune_batch(ctx, exp, samples, best_blocks_per_cu, best_ms);
 }
 
+namespace {
+int activate_host(ostft_ctx* ctx, const ostft_params& p, const float* in, uint64_t full_size, float* out,
+                  uint64_t out_cap, uint64_t* out_len, uint64_t hop, uint64_t frames);
+}
+
 int ostft_activate(ostft_ctx* ctx, const float* in, uint64_t full_size, int32_t exp, float overlap,
                    const char* options, float* out, uint64_t out_cap, uint64_t* out_len)
 {
@@ -595,8 +621,28 @@ int ostft_activate(ostft_ctx* ctx, const float* in, uint64_t full_size, int32_t 
     int rc = ostft_parse_options(options, exp, overlap, &p);
     if (rc != OSTFT_OK)
         return fail(ctx, rc, "--gaussian_window");
+    return activate_host(ctx, p, in, full_size, out, out_cap, out_len, 0, 0);
+}
+
+int ostft_activate_geometry(ostft_ctx* ctx, const ostft_params* params, const float* in, uint64_t full_size,
+                            uint64_t hop, uint64_t frames, float* out, uint64_t out_cap, uint64_t* out_len)
+{
+    if (!ctx || !params || !out || (!in && full_size > 0))
+        return fail(ctx, OSTFT_ERR_INVALID_ARG, "null argument");
+    if (!hop || !frames)
+        return fail(ctx, OSTFT_ERR_INVALID_ARG, "explicit geometry needs hop >= 1 and frames >= 1");
+    return activate_host(ctx, *params, in, full_size, out, out_cap, out_len, hop, frames);
+}
+
+}  // extern "C"
+
+namespace {
+int activate_host(ostft_ctx* ctx, const ostft_params& p, const float* in, uint64_t full_size, float* out,
+                  uint64_t out_cap, uint64_t* out_len, uint64_t hop, uint64_t frames)
+{
+    int rc;
     Plan plan;
-    rc = make_plan(&p, full_size, &plan, ctx);
+    rc = make_plan(&p, full_size, &plan, ctx, hop, frames);
     if (rc != OSTFT_OK)
         return rc;
     if (out_cap < plan.g.out_floats)
@@ -631,7 +677,7 @@ int ostft_activate(ostft_ctx* ctx, const float* in, uint64_t full_size, int32_t 
         if (e != hipSuccess)
             return hip_fail(ctx, e, "H2D copy");
     }
-    rc = ostft_run_device(ctx, &p, ctx->d_in, full_size, ctx->d_out, ctx->d_out_cap, ctx->stream);
+    rc = run_device_geometry(ctx, &p, ctx->d_in, full_size, ctx->d_out, ctx->d_out_cap, ctx->stream, hop, frames);
     if (rc != OSTFT_OK)
         return rc;
     e = hipMemcpyAsync(out, ctx->d_out, sizeof(float) * plan.g.out_floats, hipMemcpyDeviceToHost, ctx->stream);
@@ -643,5 +689,4 @@ int ostft_activate(ostft_ctx* ctx, const float* in, uint64_t full_size, int32_t 
         *out_len = plan.g.out_floats;
     return OSTFT_OK;
 }
-
-}  // extern "C"
+}  // namespace
