@@ -62,6 +62,7 @@ __device__ __forceinline__ float magnitude(float2 x2)
 }
 __device__ __forceinline__ float power_of(float2 x2) { return 0.25f * (x2.x * x2.x + x2.y * x2.y); }
 __device__ __forceinline__ float2 value_of(float2 x2) { return make_float2(0.5f * x2.x, 0.5f * x2.y); }
+__device__ __forceinline__ float2 conj_value_of(float2 x2) { return make_float2(0.5f * x2.x, -0.5f * x2.y); }
 
 // Write the outputs of bin k of frame f (frame length N = 2M) from the
 // doubled pair b.

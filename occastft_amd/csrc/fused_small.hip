@@ -44,6 +44,12 @@ namespace ostft {
 #ifndef OSTFT_DIRECT_LOG_M
 #define OSTFT_DIRECT_LOG_M 99
 #endif
+// complex sizes from 2^OSTFT_SYM_LOG_M on take the real N-value output
+// layouts from half the bins, by the conjugate symmetry of a real frame's
+// spectrum (the complex layout always does)
+#ifndef OSTFT_SYM_LOG_M
+#define OSTFT_SYM_LOG_M 6
+#endif
 // Minimum waves per SIMD the persistent batch kernel is compiled for: 4 (a
 // 128-register budget, which LDS allows anyway) where that compiles without
 // spills, else 1 (the compiler's choice).
@@ -187,34 +193,53 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
     PassTwiddles<LOG_M> ptw;
     if constexpr (XF == XF_FP32)
         load_pass_twiddles<LOG_M>(ptw, a.tw, t);
-    // epilogue: the thread handles BPT consecutive bins per iteration, BPT = 4
-    // for the real layouts (one float4 per half) and 2 for the complex layout
-    // (one float4 of two complex values per half), so that every store
-    // instruction of a wave covers a contiguous 1 KB
+    // epilogue of the N-value layouts (COMPAT_POWER, MAGNITUDE, HALF_COMPLEX):
+    // a real frame's spectrum has X[N-k] = conj X[k], so bins k0 .. k0+B of
+    // the packed spectrum give positions k0 .. k0+B-1 and M+k0 .. M+k0+B-1
+    // and, conjugated, the B positions below M-k0 and below N-k0.  A thread
+    // takes B = 2 bins per iteration for the complex layout (one float4 of two
+    // complex values per block) and B = 4 for the real layouts (one float4 per
+    // block), so every store instruction of a wave covers contiguous memory.
+    // The real layouts of the smallest window, whose blocks would be half a
+    // cache line, take 4 bins per iteration for positions k0.. and M+k0..
+    // only.
     const int mode = a.out_mode;
-    const uint32_t BPT = mode == OUT_HALF_COMPLEX ? 2u : 4u;
+    constexpr bool SYM_REAL = LOG_M >= OSTFT_SYM_LOG_M;
+    constexpr int QPF = M / 8, QITER = (G * QPF) / WG;  // real layouts: quads
+    constexpr int PPF = M / 4, PITER = (G * PPF) / WG;  // complex layout: pairs
+    static_assert(QITER * WG == G * QPF && PITER * WG == G * PPF, "blocks divide evenly over the workgroup");
+    constexpr int OITER4 = (G * M) / (WG * 4);          // real layouts without the symmetry
     // (the N/2+1-bin layouts take two consecutive bins of the flattened
     // [frame][bin] index per thread and iteration)
-    constexpr int OITER4 = (G * M) / (WG * 4);
-    constexpr int OITER2 = 2 * OITER4;
     constexpr int OITERP = (G * (M + 1) + 2 * WG - 1) / (2 * WG);
-    constexpr int NEB = OITERP > OITER2 ? OITERP : OITER2;
-    const int oiter = mode == OUT_HALF_COMPLEX ? OITER2 : OITER4;
+    constexpr int NEB = OITERP > PITER ? (OITERP > OITER4 ? OITERP : OITER4) : (PITER > OITER4 ? PITER : OITER4);
     const float2* __restrict__ twn = a.tw_n;
     const bool vec_out = mode == OUT_COMPAT_POWER || mode == OUT_MAGNITUDE || mode == OUT_HALF_COMPLEX;
     float2 eb[NEB];  // W_N^k0 for each epilogue iteration
-    float2 ec1 = make_float2(1.0f, 0.0f), ec2 = ec1, ec3 = ec1;
+    float2 ec1 = make_float2(1.0f, 0.0f), ec2 = ec1, ec3 = ec1, ec4 = ec1;
 #pragma unroll
     for (int it = 0; it < NEB; ++it)
         eb[it] = ec1;
-    if (vec_out && a.combine) {
+    if (mode == OUT_HALF_COMPLEX && a.combine) {
 #pragma unroll
-        for (int it = 0; it < OITER2; ++it)
-            if (it < oiter)
-                eb[it] = twn[((tid + it * WG) * BPT) & (M - 1)];
+        for (int it = 0; it < PITER; ++it)
+            eb[it] = twn[2 * ((tid + it * WG) & (PPF - 1))];
+        ec1 = twn[1];
+        ec2 = twn[2];
+    } else if (vec_out && a.combine) {
+        if constexpr (SYM_REAL) {
+#pragma unroll
+            for (int it = 0; it < QITER; ++it)
+                eb[it] = twn[4 * ((tid + it * WG) & (QPF - 1))];
+        } else {
+#pragma unroll
+            for (int it = 0; it < OITER4; ++it)
+                eb[it] = twn[(4 * (tid + it * WG)) & (M - 1)];
+        }
         ec1 = twn[1];
         ec2 = twn[2];
         ec3 = twn[3];
+        ec4 = twn[4];
     } else if (a.combine) {
 #pragma unroll
         for (int it = 0; it < OITERP; ++it) {
@@ -238,32 +263,61 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
         for (int m = 0; m < 16; ++m)
             v[m] = w2 ? make_float2(xd[m].x * wv[m].x, xd[m].y * wv[m].y) : xd[m];
     } else {
-        // stage the span in LDS
+        // stage the span in LDS; 4*WG is a multiple of 128, so vector i and
+        // vector i + WG are a fixed distance apart in the padded layout
+        constexpr uint32_t SROW = 4 * WG + WG / 8;
+        float* const dst = sf + spad(4 * tid);
 #pragma unroll
         for (int it = 0; it < ITER; ++it) {
-            const uint32_t i = tid + it * WG;
-            if (i < nvec)
-                *reinterpret_cast<float4*>(sf + spad(4 * i)) = x[it];
+            if (tid + it * WG < nvec)
+                *reinterpret_cast<float4*>(dst + it * SROW) = x[it];
         }
         __syncthreads();
 
         const uint32_t q0 = (uint32_t)(grp * hop + (s_begin - s_al));
         // an even q0 keeps every (x[2n], x[2n+1]) pair inside one padded row
         const bool pair_ok = (q0 & 1u) == 0;
+        if constexpr (T >= 64) {
+            // a wave lies inside one group, and 2T is a multiple of 128:
+            // sample q0 + 2(t + mT) is staged at spad(q0 + 2t) + m (2T + T/16)
+            constexpr uint32_t SSTEP = 2 * T + T / 16;
 #pragma unroll
-        for (int m = 0; m < 16; ++m) {
-            const uint32_t n = t + m * T;
-            float2 xv = make_float2(0.0f, 0.0f);
+            for (int m = 0; m < 16; ++m)
+                v[m] = make_float2(0.0f, 0.0f);
             if (active) {
-                const uint32_t q = q0 + 2 * n;
-                if (OSTFT_PAIR_LDS && pair_ok)
-                    xv = *reinterpret_cast<const float2*>(sf + spad(q));
-                else
-                    xv = make_float2(sf[spad(q)], sf[spad(q + 1)]);
+                const float* const b0 = sf + spad(q0 + 2 * t);
+                if (OSTFT_PAIR_LDS && pair_ok) {
+#pragma unroll
+                    for (int m = 0; m < 16; ++m)
+                        v[m] = *reinterpret_cast<const float2*>(b0 + m * SSTEP);
+                } else {
+                    const float* const b1 = sf + spad(q0 + 2 * t + 1);
+#pragma unroll
+                    for (int m = 0; m < 16; ++m)
+                        v[m] = make_float2(b0[m * SSTEP], b1[m * SSTEP]);
+                }
             }
-            if (w2)
-                xv = make_float2(xv.x * wv[m].x, xv.y * wv[m].y);
-            v[m] = xv;
+            if (w2) {
+#pragma unroll
+                for (int m = 0; m < 16; ++m)
+                    v[m] = make_float2(v[m].x * wv[m].x, v[m].y * wv[m].y);
+            }
+        } else {
+#pragma unroll
+            for (int m = 0; m < 16; ++m) {
+                const uint32_t n = t + m * T;
+                float2 xv = make_float2(0.0f, 0.0f);
+                if (active) {
+                    const uint32_t q = q0 + 2 * n;
+                    if (OSTFT_PAIR_LDS && pair_ok)
+                        xv = *reinterpret_cast<const float2*>(sf + spad(q));
+                    else
+                        xv = make_float2(sf[spad(q)], sf[spad(q + 1)]);
+                }
+                if (w2)
+                    xv = make_float2(xv.x * wv[m].x, xv.y * wv[m].y);
+                v[m] = xv;
+            }
         }
         __syncthreads();  // the FFT reuses the staging buffer
     }
@@ -295,26 +349,76 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
 
     // ---- 5. last radix-2 stage + output, the whole block's output at once ----
     float* __restrict__ out = a.out;
+    // mirrored blocks: X[M-k] = conj X[M+k] and X[N-k] = conj X[k] after the
+    // last radix-2 stage; for [E, O] (no last stage) E[M-k] = conj E[k] and
+    // O[M-k] = conj O[k] go to positions M-k and N-k instead
+    const uint32_t hm = a.combine ? (uint32_t)M : (uint32_t)N;  // mirror of the M+k block
+    const uint32_t lm = a.combine ? (uint32_t)N : (uint32_t)M;  // mirror of the k block
     if (mode == OUT_HALF_COMPLEX) {
-        const uint32_t nbins = nf * (uint32_t)M;
         const uint64_t lim = a.out_limit;  // complex values
         float4* o = reinterpret_cast<float4*>(out);
 #pragma unroll
-        for (int it = 0; it < OITER2; ++it) {
-            const uint32_t e = (tid + it * WG) * 2;
-            if (e >= nbins)
+        for (int it = 0; it < PITER; ++it) {
+            const uint32_t i = tid + it * WG;
+            const uint32_t g = i / PPF, k0 = 2 * (i & (PPF - 1));
+            if (g >= nf)
                 break;
-            const uint32_t g = e >> LOG_M, k0 = e & (M - 1);
+            // Z[k0], Z[k0+1] and Z[M-k0-2], Z[M-k0-1] each share a row
+            // of the padded layout
             const float2* z = lds + g * PADM;
-            const BinPair b0 = finish_bin(z[pad(k0)], z[pad((M - k0) & (M - 1))], eb[it], a.combine, false);
-            const BinPair b1 = finish_bin(z[pad(k0 + 1)], z[pad(M - k0 - 1)], cmul(eb[it], ec1), a.combine, false);
+            const float2* zf = z + pad(k0);
+            const float2* zm = z + pad((uint32_t)M - k0 - 2);
+            const float2 w1 = cmul(eb[it], ec1), w2 = cmul(eb[it], ec2);
+            const BinPair b0 = finish_bin(zf[0], z[pad(((uint32_t)M - k0) & (M - 1))], eb[it], a.combine, false);
+            const BinPair b1 = finish_bin(zf[1], zm[1], w1, a.combine, false);
+            const BinPair b2 = finish_bin(z[pad(k0 + 2)], zm[0], w2, a.combine, false);
             const uint64_t base = (f0 + g) * (uint64_t)N;
-            if (base + k0 < lim)
-                nt_store4(make_float4(0.5f * b0.lo.x, 0.5f * b0.lo.y, 0.5f * b1.lo.x, 0.5f * b1.lo.y),
-                          o + (base + k0) / 2);
-            if (base + M + k0 < lim)
-                nt_store4(make_float4(0.5f * b0.hi.x, 0.5f * b0.hi.y, 0.5f * b1.hi.x, 0.5f * b1.hi.y),
-                          o + (base + M + k0) / 2);
+            const uint64_t c0 = base + k0, c1 = base + M + k0, c2 = base + hm - k0 - 2, c3 = base + lm - k0 - 2;
+            const float2 v0 = value_of(b0.lo), v1 = value_of(b1.lo);
+            const float2 h0 = value_of(b0.hi), h1 = value_of(b1.hi);
+            const float2 mh2 = conj_value_of(b2.hi), mh1 = conj_value_of(b1.hi);
+            const float2 ml2 = conj_value_of(b2.lo), ml1 = conj_value_of(b1.lo);
+            if (c0 < lim)
+                nt_store4(make_float4(v0.x, v0.y, v1.x, v1.y), o + c0 / 2);
+            if (c1 < lim)
+                nt_store4(make_float4(h0.x, h0.y, h1.x, h1.y), o + c1 / 2);
+            if (c2 < lim)
+                nt_store4(make_float4(mh2.x, mh2.y, mh1.x, mh1.y), o + c2 / 2);
+            if (c3 < lim)
+                nt_store4(make_float4(ml2.x, ml2.y, ml1.x, ml1.y), o + c3 / 2);
+        }
+    } else if (vec_out && SYM_REAL) {
+#pragma unroll
+        for (int it = 0; it < QITER; ++it) {
+            const uint32_t i = tid + it * WG;
+            const uint32_t g = i / QPF, k0 = 4 * (i & (QPF - 1));
+            if (g >= nf)
+                break;
+            // Z[k0 .. k0+3] and Z[M-k0-4 .. M-k0-1] each share a row of
+            // the padded layout
+            const float2* z = lds + g * PADM;
+            const float2* zf = z + pad(k0);
+            const float2* zm = z + pad((uint32_t)M - k0 - 4);
+            const float2 wk[5] = {eb[it], cmul(eb[it], ec1), cmul(eb[it], ec2), cmul(eb[it], ec3),
+                                  cmul(eb[it], ec4)};
+            BinPair b[5];
+            b[0] = finish_bin(zf[0], z[pad(((uint32_t)M - k0) & (M - 1))], wk[0], a.combine, false);
+#pragma unroll
+            for (int j = 1; j < 4; ++j)
+                b[j] = finish_bin(zf[j], zm[4 - j], wk[j], a.combine, false);
+            b[4] = finish_bin(z[pad(k0 + 4)], zm[0], wk[4], a.combine, false);
+            // the value of a conjugate is the value itself
+            float lo[5], hi[5];
+#pragma unroll
+            for (int j = 0; j < 5; ++j) {
+                lo[j] = mode == OUT_COMPAT_POWER ? compat_power(b[j].lo) : magnitude(b[j].lo);
+                hi[j] = mode == OUT_COMPAT_POWER ? compat_power(b[j].hi) : magnitude(b[j].hi);
+            }
+            float4* o = reinterpret_cast<float4*>(out + (f0 + g) * (uint64_t)N);
+            nt_store4(make_float4(lo[0], lo[1], lo[2], lo[3]), o + k0 / 4);
+            nt_store4(make_float4(hi[0], hi[1], hi[2], hi[3]), o + (M + k0) / 4);
+            nt_store4(make_float4(hi[4], hi[3], hi[2], hi[1]), o + (hm - k0 - 4) / 4);
+            nt_store4(make_float4(lo[4], lo[3], lo[2], lo[1]), o + (lm - k0 - 4) / 4);
         }
     } else if (vec_out) {
         const uint32_t nbins = nf * (uint32_t)M;
@@ -326,26 +430,17 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
             const uint32_t g = e >> LOG_M, k0 = e & (M - 1);
             const float2* z = lds + g * PADM;
             const float2 wk[4] = {eb[it], cmul(eb[it], ec1), cmul(eb[it], ec2), cmul(eb[it], ec3)};
-            BinPair b[4];
+            float lo[4], hi[4];
 #pragma unroll
             for (uint32_t j = 0; j < 4; ++j) {
                 const uint32_t k = k0 + j, km = (M - k) & (M - 1);
-                b[j] = finish_bin(z[pad(k)], z[pad(km)], wk[j], a.combine, false);
+                const BinPair b = finish_bin(z[pad(k)], z[pad(km)], wk[j], a.combine, false);
+                lo[j] = mode == OUT_COMPAT_POWER ? compat_power(b.lo) : magnitude(b.lo);
+                hi[j] = mode == OUT_COMPAT_POWER ? compat_power(b.hi) : magnitude(b.hi);
             }
-            const uint64_t base = (f0 + g) * (uint64_t)N;
-            float4 lo, hi;
-            if (mode == OUT_COMPAT_POWER) {
-                lo = make_float4(compat_power(b[0].lo), compat_power(b[1].lo), compat_power(b[2].lo),
-                                 compat_power(b[3].lo));
-                hi = make_float4(compat_power(b[0].hi), compat_power(b[1].hi), compat_power(b[2].hi),
-                                 compat_power(b[3].hi));
-            } else {
-                lo = make_float4(magnitude(b[0].lo), magnitude(b[1].lo), magnitude(b[2].lo), magnitude(b[3].lo));
-                hi = make_float4(magnitude(b[0].hi), magnitude(b[1].hi), magnitude(b[2].hi), magnitude(b[3].hi));
-            }
-            float4* o = reinterpret_cast<float4*>(out + base);
-            nt_store4(lo, o + k0 / 4);
-            nt_store4(hi, o + (M + k0) / 4);
+            float4* o = reinterpret_cast<float4*>(out + (f0 + g) * (uint64_t)N);
+            nt_store4(make_float4(lo[0], lo[1], lo[2], lo[3]), o + k0 / 4);
+            nt_store4(make_float4(hi[0], hi[1], hi[2], hi[3]), o + (M + k0) / 4);
         }
     } else {
         // POWER / HALF_SPECTRUM: M+1 values per frame; two consecutive values

@@ -23,6 +23,9 @@
 #ifndef OSTFT_BIG_LOG_M
 #define OSTFT_BIG_LOG_M 99
 #endif
+#ifndef OSTFT_DPP_SUM
+#define OSTFT_DPP_SUM 1
+#endif
 
 namespace ostft {
 
@@ -57,16 +60,50 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t bid, uint32_t nb)
     return xcd < r ? xcd * (q + 1u) + idx : r * (q + 1u) + (xcd - r) * q + idx;
 }
 
+// x plus its value in the partner lane of a butterfly step over aligned
+// blocks of 2*OFF lanes.  Every lane of each aligned OFF-block holds the same
+// value when the step runs, so any lane of the other half is a valid partner:
+// quad permutations for 1 and 2, the half-row and row mirrors (lane i with
+// 7-i, 15-i) for 4 and 8 -- all folded into the add as DPP operands -- a
+// swizzle for 16 and a lane permute for 32.
+template <int OFF>
+__device__ __forceinline__ float lane_pair_add(float x)
+{
+    const int xi = __builtin_bit_cast(int, x);
+    int p;
+    if constexpr (OFF == 1)
+        p = __builtin_amdgcn_update_dpp(0, xi, 0xB1, 0xF, 0xF, false);  // quad_perm [1,0,3,2]
+    else if constexpr (OFF == 2)
+        p = __builtin_amdgcn_update_dpp(0, xi, 0x4E, 0xF, 0xF, false);  // quad_perm [2,3,0,1]
+    else if constexpr (OFF == 4)
+        p = __builtin_amdgcn_update_dpp(0, xi, 0x141, 0xF, 0xF, false);  // row_half_mirror
+    else if constexpr (OFF == 8)
+        p = __builtin_amdgcn_update_dpp(0, xi, 0x140, 0xF, 0xF, false);  // row_mirror
+    else if constexpr (OFF == 16)
+        p = __builtin_amdgcn_ds_swizzle(xi, 0x401F);  // bit mode: lane ^ 16 within 32
+    else
+        p = __builtin_bit_cast(int, __shfl_xor(x, OFF, 64));
+    return x + __builtin_bit_cast(float, p);
+}
+
 // Sum over the T consecutive lanes of a group (all threads must call it).
 template <int T, int WG>
 __device__ __forceinline__ float2 group_sum(float2 s, float2* red, uint32_t tid)
 {
     constexpr int W = T < 64 ? T : 64;
+#if OSTFT_DPP_SUM
+    cfor<0, ilog2(W)>([&](auto b) {
+        constexpr int OFF = 1 << decltype(b)::value;
+        s.x = lane_pair_add<OFF>(s.x);
+        s.y = lane_pair_add<OFF>(s.y);
+    });
+#else
 #pragma unroll
     for (int off = W / 2; off > 0; off >>= 1) {
         s.x += __shfl_xor(s.x, off, 64);
         s.y += __shfl_xor(s.y, off, 64);
     }
+#endif
     if constexpr (T > 64) {
         constexpr int WPG = T / 64;
         const uint32_t wave = tid >> 6;
