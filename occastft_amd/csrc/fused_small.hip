@@ -277,47 +277,42 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
         const uint32_t q0 = (uint32_t)(grp * hop + (s_begin - s_al));
         // an even q0 keeps every (x[2n], x[2n+1]) pair inside one padded row
         const bool pair_ok = (q0 & 1u) == 0;
-        if constexpr (T >= 64) {
-            // a wave lies inside one group, and 2T is a multiple of 128:
-            // sample q0 + 2(t + mT) is staged at spad(q0 + 2t) + m (2T + T/16)
-            constexpr uint32_t SSTEP = 2 * T + T / 16;
+        // sample q0 + 2(t + mT): with m = KB a + b, 2T m = 2T b + 128 a (T < 64,
+        // KB = 64/T) or m (2T), a multiple of 128 (T >= 64, KB = 1), so it is
+        // staged at spad(q0 + 2t + 2T b) + a ASTEP -- KB base addresses and
+        // immediate offsets instead of one padded index per value
+        constexpr int KB = T >= 64 ? 1 : (64 / T < 16 ? 64 / T : 16);
+        constexpr uint32_t ASTEP = T >= 64 ? 2 * T + T / 16 : 132;
+#pragma unroll
+        for (int m = 0; m < 16; ++m)
+            v[m] = make_float2(0.0f, 0.0f);
+        if (active) {
+            const uint32_t qt = q0 + 2 * t;
+            if (OSTFT_PAIR_LDS && pair_ok) {
+                cfor<0, KB>([&](auto bb) {
+                    constexpr int B = decltype(bb)::value;
+                    const float* const base = sf + spad(qt + 2 * T * B);
+                    cfor<0, 16 / KB>([&](auto aa) {
+                        constexpr int A = decltype(aa)::value;
+                        v[KB * A + B] = *reinterpret_cast<const float2*>(base + A * ASTEP);
+                    });
+                });
+            } else {
+                cfor<0, KB>([&](auto bb) {
+                    constexpr int B = decltype(bb)::value;
+                    const float* const b0 = sf + spad(qt + 2 * T * B);
+                    const float* const b1 = sf + spad(qt + 2 * T * B + 1);
+                    cfor<0, 16 / KB>([&](auto aa) {
+                        constexpr int A = decltype(aa)::value;
+                        v[KB * A + B] = make_float2(b0[A * ASTEP], b1[A * ASTEP]);
+                    });
+                });
+            }
+        }
+        if (w2) {
 #pragma unroll
             for (int m = 0; m < 16; ++m)
-                v[m] = make_float2(0.0f, 0.0f);
-            if (active) {
-                const float* const b0 = sf + spad(q0 + 2 * t);
-                if (OSTFT_PAIR_LDS && pair_ok) {
-#pragma unroll
-                    for (int m = 0; m < 16; ++m)
-                        v[m] = *reinterpret_cast<const float2*>(b0 + m * SSTEP);
-                } else {
-                    const float* const b1 = sf + spad(q0 + 2 * t + 1);
-#pragma unroll
-                    for (int m = 0; m < 16; ++m)
-                        v[m] = make_float2(b0[m * SSTEP], b1[m * SSTEP]);
-                }
-            }
-            if (w2) {
-#pragma unroll
-                for (int m = 0; m < 16; ++m)
-                    v[m] = make_float2(v[m].x * wv[m].x, v[m].y * wv[m].y);
-            }
-        } else {
-#pragma unroll
-            for (int m = 0; m < 16; ++m) {
-                const uint32_t n = t + m * T;
-                float2 xv = make_float2(0.0f, 0.0f);
-                if (active) {
-                    const uint32_t q = q0 + 2 * n;
-                    if (OSTFT_PAIR_LDS && pair_ok)
-                        xv = *reinterpret_cast<const float2*>(sf + spad(q));
-                    else
-                        xv = make_float2(sf[spad(q)], sf[spad(q + 1)]);
-                }
-                if (w2)
-                    xv = make_float2(xv.x * wv[m].x, xv.y * wv[m].y);
-                v[m] = xv;
-            }
+                v[m] = make_float2(v[m].x * wv[m].x, v[m].y * wv[m].y);
         }
         __syncthreads();  // the FFT reuses the staging buffer
     }
