@@ -90,7 +90,9 @@ __global__ void __launch_bounds__(MP_THREADS) first_pass_kernel(const PassArgs a
     PassTwiddles<LOG_R> tws;
     load_pass_twiddles<LOG_R>(tws, a.tw_r, tid % T);
 
-    // rows of CB contiguous packed samples z[n] = x[2n] + i x[2n+1], windowed
+    // rows of CB contiguous packed samples z[n] = x[2n] + i x[2n+1], windowed;
+    // 8-byte loads of the pairs where the frame start keeps them aligned
+    const bool pair_ok = (reinterpret_cast<uintptr_t>(a.sig + P * a.hop) & 7u) == 0;
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
         const uint32_t e = tid + MP_THREADS * j;
@@ -98,9 +100,14 @@ __global__ void __launch_bounds__(MP_THREADS) first_pass_kernel(const PassArgs a
         const uint32_t n = g0 + c + m * stride;
         const uint64_t s0 = P * a.hop + 2ull * n;
         const float2 w = a.win ? reinterpret_cast<const float2*>(a.win)[n] : make_float2(1.0f, 1.0f);
-        const float x0 = s0 < a.L ? a.sig[s0] : 0.0f;
-        const float x1 = s0 + 1 < a.L ? a.sig[s0 + 1] : 0.0f;
-        lds[c * PADR + pad(m)] = make_float2(x0 * w.x, x1 * w.y);
+        float2 xv;
+        if (pair_ok && s0 + 2 <= a.L) {
+            xv = *reinterpret_cast<const float2*>(a.sig + s0);
+        } else {
+            xv.x = s0 < a.L ? a.sig[s0] : 0.0f;
+            xv.y = s0 + 1 < a.L ? a.sig[s0 + 1] : 0.0f;
+        }
+        lds[c * PADR + pad(m)] = make_float2(xv.x * w.x, xv.y * w.y);
     }
     __syncthreads();
 
@@ -129,6 +136,7 @@ struct LastArgs {
     const float2* tw_fine;    // W_M^j, j < 2^fine_bits
     const float2* tw_coarse;  // W_M^(j << fine_bits)
     const float2* tw_n;       // W_N^k, k < M
+    const float2* tw_2r;      // W_2R^q = W_N^(q Ns), q < R
     float* out;
     uint64_t frames, out_limit;
     uint32_t fine_bits, log_m;
@@ -162,18 +170,28 @@ __global__ void __launch_bounds__(MP_THREADS) last_pass_kernel(const LastArgs a)
         const uint32_t h = c - CB;
         return (b == 0 && h == 0) ? Ns / 2 : Ns - (b * CB + h);
     };
+    // The thread's 16 values share one column: c = tid mod CB2 (butterfly g),
+    // rows m = m0 + T j.  Inter-pass twiddle W_M^{g m} (g m < M): from the
+    // two-level FP64 table at every fourth row, and by products with
+    // W_M^{g T} in between (at most three chained roundings).
     const uint32_t fmask = (1u << a.fine_bits) - 1u;
+    auto twiddle = [&](uint32_t ex) {
+        return cmul(a.tw_fine[ex & fmask], a.tw_coarse[ex >> a.fine_bits]);
+    };
     const float2* src = a.src + P * M;
+    const uint32_t cc = tid % CB2, m0 = tid / CB2;
+    const uint32_t gc = butterfly(cc);
+    const float2 step = twiddle(gc * T);
+    float2 anchor[4];
+#pragma unroll
+    for (int j = 0; j < 16; j += 4)
+        anchor[j / 4] = twiddle(gc * (m0 + T * j));
+    float2 tw = anchor[0];
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
-        const uint32_t e = tid + MP_THREADS * j;
-        const uint32_t c = e % CB2, m = e / CB2;
-        const uint32_t g = butterfly(c);
-        float2 x = src[g + (uint64_t)m * Ns];
-        const uint32_t ex = g * m;  // < M: W_{Ns R}^{g m} = W_M^{g m}
-        if (ex)
-            x = cmul(x, cmul(a.tw_fine[ex & fmask], a.tw_coarse[ex >> a.fine_bits]));
-        lds[c * PADR + pad(m)] = x;
+        const uint32_t m = m0 + T * j;
+        tw = (j % 4 == 0) ? anchor[j / 4] : cmul(tw, step);
+        lds[cc * PADR + pad(m)] = cmul(src[gc + (uint64_t)m * Ns], tw);
     }
     __syncthreads();
 
@@ -188,34 +206,21 @@ __global__ void __launch_bounds__(MP_THREADS) last_pass_kernel(const LastArgs a)
     // column c, row q now holds Z[k], k = butterfly(c) + q Ns
 
     // split: Z[M - k] is column mirror(c), row R - 1 - q (row (R - q) mod R
-    // for butterfly 0); W_N^k for all 16 of the thread's bins first
-    uint32_t kk[16];
-    float2 wk[16];
+    // for butterfly 0).  The thread's bins are k = g + q Ns, q = m0 + T j, of
+    // its column c = cc: W_N^k = W_N^g W_2R^q.
+    const float2 wg = a.combine ? a.tw_n[gc] : make_float2(1.0f, 0.0f);
+    uint32_t mc;
+    if (gc == 0 || (gc == Ns / 2 && b == 0))
+        mc = cc;
+    else
+        mc = cc < (uint32_t)CB ? cc + CB : cc - CB;
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
-        const uint32_t e = tid + MP_THREADS * j;
-        const uint32_t c = e % CB2, q = e / CB2;
-        kk[j] = butterfly(c) + q * Ns;
-        wk[j] = a.combine ? a.tw_n[kk[j]] : make_float2(1.0f, 0.0f);
-    }
-#pragma unroll
-    for (int j = 0; j < 16; ++j) {
-        const uint32_t e = tid + MP_THREADS * j;
-        const uint32_t c = e % CB2, q = e / CB2;
-        const uint32_t g = butterfly(c);
-        uint32_t mc, mq;
-        if (g == 0) {
-            mc = c;
-            mq = (R - q) & (R - 1);
-        } else if (g == Ns / 2 && b == 0) {
-            mc = c;
-            mq = R - 1 - q;
-        } else {
-            mc = c < (uint32_t)CB ? c + CB : c - CB;
-            mq = R - 1 - q;
-        }
-        const uint32_t k = kk[j];
-        const BinPair bp = finish_bin(lds[c * PADR + pad(q)], lds[mc * PADR + pad(mq)], wk[j], a.combine,
+        const uint32_t q = m0 + T * j;
+        const uint32_t mq = gc == 0 ? (R - q) & (R - 1) : R - 1 - q;
+        const uint32_t k = gc + q * Ns;
+        const float2 wk = a.combine ? cmul(wg, a.tw_2r[q]) : wg;
+        const BinPair bp = finish_bin(lds[cc * PADR + pad(q)], lds[mc * PADR + pad(mq)], wk, a.combine,
                                       a.dc && k == 0);
         write_bin(a.out, a.out_mode, P, (uint32_t)M, k, bp, a.out_limit);
     }
@@ -277,6 +282,9 @@ int run_multipass(ostft_ctx* ctx, int log_m, const FusedArgs& a, hipStream_t s)
     const float2* tw_r1 = ctx_twiddles(ctx, (int64_t)1 << lr[1], (int64_t)1 << lr[1], &rc);
     if (!tw_r1)
         return rc;
+    const float2* tw_2r1 = ctx_twiddles(ctx, (int64_t)2 << lr[1], (int64_t)1 << lr[1], &rc);
+    if (!tw_2r1)
+        return rc;
 
     PassArgs p;
     p.sig = a.in;
@@ -304,6 +312,7 @@ int run_multipass(ostft_ctx* ctx, int log_m, const FusedArgs& a, hipStream_t s)
     la.tw_fine = tw_fine;
     la.tw_coarse = tw_coarse;
     la.tw_n = a.tw_n;
+    la.tw_2r = tw_2r1;
     la.out = a.out;
     la.frames = a.frames;
     la.out_limit = a.out_limit;
