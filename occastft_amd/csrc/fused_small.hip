@@ -209,10 +209,13 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
     constexpr int PPF = M / 4, PITER = (G * PPF) / WG;  // complex layout: pairs
     static_assert(QITER * WG == G * QPF && PITER * WG == G * PPF, "blocks divide evenly over the workgroup");
     constexpr int OITER4 = (G * M) / (WG * 4);          // real layouts without the symmetry
-    // (the N/2+1-bin layouts take two consecutive bins of the flattened
-    // [frame][bin] index per thread and iteration)
+    // (the N/2+1-bin layouts: slots k <= M/2 of each frame, one per thread
+    // and iteration; at the smallest window two consecutive values of the
+    // flattened [frame][bin] index)
+    constexpr int OITERS = (G * (M / 2 + 1) + WG - 1) / WG;
     constexpr int OITERP = (G * (M + 1) + 2 * WG - 1) / (2 * WG);
-    constexpr int NEB = OITERP > PITER ? (OITERP > OITER4 ? OITERP : OITER4) : (PITER > OITER4 ? PITER : OITER4);
+    constexpr int OITERM = OITERS > OITERP ? OITERS : OITERP;
+    constexpr int NEB = OITERM > PITER ? (OITERM > OITER4 ? OITERM : OITER4) : (PITER > OITER4 ? PITER : OITER4);
     const float2* __restrict__ twn = a.tw_n;
     const bool vec_out = mode == OUT_COMPAT_POWER || mode == OUT_MAGNITUDE || mode == OUT_HALF_COMPLEX;
     float2 eb[NEB];  // W_N^k0 for each epilogue iteration
@@ -241,15 +244,24 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
         ec3 = twn[3];
         ec4 = twn[4];
     } else if (a.combine) {
+        if constexpr (SYM_REAL) {
 #pragma unroll
-        for (int it = 0; it < OITERP; ++it) {
-            const uint32_t e1 = 2u * (tid + it * WG);
-            if (e1 < (uint32_t)(G * (M + 1))) {
-                const uint32_t k1 = e1 % (uint32_t)(M + 1);
-                eb[it] = twn[k1 == (uint32_t)M ? 0u : k1];
+            for (int it = 0; it < OITERS; ++it) {
+                const uint32_t e = tid + it * WG;
+                if (e < (uint32_t)(G * (M / 2 + 1)))
+                    eb[it] = twn[e % (uint32_t)(M / 2 + 1)];
             }
+        } else {
+#pragma unroll
+            for (int it = 0; it < OITERP; ++it) {
+                const uint32_t e1 = 2u * (tid + it * WG);
+                if (e1 < (uint32_t)(G * (M + 1))) {
+                    const uint32_t k1 = e1 % (uint32_t)(M + 1);
+                    eb[it] = twn[k1 == (uint32_t)M ? 0u : k1];
+                }
+            }
+            ec1 = twn[1];
         }
-        ec1 = twn[1];
     }
 
 #if OSTFT_SETPRIO
@@ -438,54 +450,93 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
             nt_store4(make_float4(hi[0], hi[1], hi[2], hi[3]), o + (M + k0) / 4);
         }
     } else {
-        // POWER / HALF_SPECTRUM: M+1 values per frame; two consecutive values
-        // of the flattened [frame][bin] index per thread and iteration, one
-        // 8-byte (POWER) or 16-byte (HALF_SPECTRUM) store where aligned
-        const uint32_t nval = nf * (uint32_t)(M + 1);
-        const uint64_t obase = f0 * (uint64_t)(M + 1);
-        const float2 one = make_float2(1.0f, 0.0f);
+        if constexpr (SYM_REAL) {
+            // POWER / HALF_SPECTRUM: M+1 values per frame.  Slot k = 0 .. M/2 of
+            // a frame gives positions k and M-k from one pair (Z[k], Z[M-k]):
+            // X[k] and X[M-k] = conj X[M+k] after the last radix-2 stage, E[k] and
+            // E[M-k] = conj E[k] for [E, O]; position M (from slot 0) is X[M] or
+            // O[0].  Consecutive slots sit on consecutive lanes, so each store
+            // instruction covers contiguous memory (ascending for k, descending
+            // for M-k).
+            constexpr uint32_t SPF = M / 2 + 1;  // slots per frame
+            const uint32_t nslot = nf * SPF;
+            const uint64_t obase = f0 * (uint64_t)(M + 1);
+            const float2 one = make_float2(1.0f, 0.0f);
 #pragma unroll
-        for (int it = 0; it < OITERP; ++it) {
-            const uint32_t e1 = 2u * (tid + it * WG);
-            if (e1 >= nval)
-                break;
-            const bool has2 = e1 + 1 < nval;
-            const uint32_t g1 = e1 / (uint32_t)(M + 1), k1 = e1 - g1 * (uint32_t)(M + 1);
-            const uint32_t kb1 = k1 == (uint32_t)M ? 0u : k1;
-            const float2 w1 = a.combine ? eb[it] : one;
-            const float2* z1 = lds + g1 * PADM;
-            const BinPair b1 = finish_bin(z1[pad(kb1)], z1[pad((M - kb1) & (M - 1))], w1, a.combine, false);
-            const float2 x1 = k1 == (uint32_t)M ? b1.hi : b1.lo;  // doubled
-            // the next value: bin k1 + 1 of the same frame, or bin 0 of the next
-            const uint32_t g2 = k1 == (uint32_t)M ? g1 + 1 : g1, k2 = k1 == (uint32_t)M ? 0u : k1 + 1;
-            const uint32_t kb2 = k2 == (uint32_t)M ? 0u : k2;
-            float2 x2 = make_float2(0.0f, 0.0f);
-            if (has2) {
-                const float2 w2 = (a.combine && kb2 != 0u) ? cmul(w1, ec1) : one;
-                const float2* z2 = lds + g2 * PADM;
-                const BinPair b2 = finish_bin(z2[pad(kb2)], z2[pad((M - kb2) & (M - 1))], w2, a.combine, false);
-                x2 = k2 == (uint32_t)M ? b2.hi : b2.lo;
-            }
-            const uint64_t o = obase + e1;
-            const bool pair = has2 && (o & 1u) == 0;
-            if (mode == OUT_POWER) {
-                const float p1 = power_of(x1), p2 = power_of(x2);
-                if (pair) {
-                    nt_store(make_float2(p1, p2), reinterpret_cast<float2*>(out + o));
+            for (int it = 0; it < OITERS; ++it) {
+                const uint32_t e = tid + it * WG;
+                if (e >= nslot)
+                    break;
+                const uint32_t g = e / SPF, k = e - g * SPF;
+                const float2* z = lds + g * PADM;
+                const BinPair b = finish_bin(z[pad(k)], z[pad(((uint32_t)M - k) & (M - 1))], a.combine ? eb[it] : one,
+                                             a.combine, false);
+                const float2 x1 = b.lo;                                   // position k (doubled)
+                const float2 x2 = k == 0 ? b.hi : (a.combine ? b.hi : b.lo);  // position M-k, before conj
+                const bool second = k != (uint32_t)(M / 2);
+                const uint64_t o = obase + (uint64_t)g * (M + 1);
+                if (mode == OUT_POWER) {
+                    nt_store(power_of(x1), out + o + k);
+                    if (second)
+                        nt_store(power_of(x2), out + o + (M - k));
                 } else {
-                    nt_store(p1, out + o);
-                    if (has2)
-                        nt_store(p2, out + o + 1);
+                    float2* o2 = reinterpret_cast<float2*>(out) + o;
+                    nt_store(value_of(x1), o2 + k);
+                    if (second)
+                        nt_store(k == 0 ? value_of(x2) : conj_value_of(x2), o2 + (M - k));
                 }
-            } else {
-                float2* o2 = reinterpret_cast<float2*>(out) + o;
-                const float2 v1 = value_of(x1), v2 = value_of(x2);
-                if (pair) {
-                    nt_store4(make_float4(v1.x, v1.y, v2.x, v2.y), reinterpret_cast<float4*>(o2));
+            }
+        } else {
+            // POWER / HALF_SPECTRUM at the smallest window: two consecutive values
+            // of the flattened [frame][bin] index per thread and iteration, one
+            // 8-byte (POWER) or 16-byte (HALF_SPECTRUM) store where aligned (the
+            // slot form's runs would be shorter than a cache line)
+            const uint32_t nval = nf * (uint32_t)(M + 1);
+            const uint64_t obase = f0 * (uint64_t)(M + 1);
+            const float2 one = make_float2(1.0f, 0.0f);
+#pragma unroll
+            for (int it = 0; it < OITERP; ++it) {
+                const uint32_t e1 = 2u * (tid + it * WG);
+                if (e1 >= nval)
+                    break;
+                const bool has2 = e1 + 1 < nval;
+                const uint32_t g1 = e1 / (uint32_t)(M + 1), k1 = e1 - g1 * (uint32_t)(M + 1);
+                const uint32_t kb1 = k1 == (uint32_t)M ? 0u : k1;
+                const float2 w1 = a.combine ? eb[it] : one;
+                const float2* z1 = lds + g1 * PADM;
+                const BinPair b1 = finish_bin(z1[pad(kb1)], z1[pad((M - kb1) & (M - 1))], w1, a.combine, false);
+                const float2 x1 = k1 == (uint32_t)M ? b1.hi : b1.lo;  // doubled
+                // the next value: bin k1 + 1 of the same frame, or bin 0 of the next
+                const uint32_t g2 = k1 == (uint32_t)M ? g1 + 1 : g1, k2 = k1 == (uint32_t)M ? 0u : k1 + 1;
+                const uint32_t kb2 = k2 == (uint32_t)M ? 0u : k2;
+                float2 x2 = make_float2(0.0f, 0.0f);
+                if (has2) {
+                    const float2 w2 = (a.combine && kb2 != 0u) ? cmul(w1, ec1) : one;
+                    const float2* z2 = lds + g2 * PADM;
+                    const BinPair b2 = finish_bin(z2[pad(kb2)], z2[pad((M - kb2) & (M - 1))], w2, a.combine, false);
+                    x2 = k2 == (uint32_t)M ? b2.hi : b2.lo;
+                }
+                const uint64_t o = obase + e1;
+                const bool pair = has2 && (o & 1u) == 0;
+                if (mode == OUT_POWER) {
+                    const float p1 = power_of(x1), p2 = power_of(x2);
+                    if (pair) {
+                        nt_store(make_float2(p1, p2), reinterpret_cast<float2*>(out + o));
+                    } else {
+                        nt_store(p1, out + o);
+                        if (has2)
+                            nt_store(p2, out + o + 1);
+                    }
                 } else {
-                    nt_store(v1, o2);
-                    if (has2)
-                        nt_store(v2, o2 + 1);
+                    float2* o2 = reinterpret_cast<float2*>(out) + o;
+                    const float2 v1 = value_of(x1), v2 = value_of(x2);
+                    if (pair) {
+                        nt_store4(make_float4(v1.x, v1.y, v2.x, v2.y), reinterpret_cast<float4*>(o2));
+                    } else {
+                        nt_store(v1, o2);
+                        if (has2)
+                            nt_store(v2, o2 + 1);
+                    }
                 }
             }
         }
