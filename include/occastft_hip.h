@@ -140,7 +140,7 @@ int ostft_run_device(ostft_ctx* ctx, const ostft_params* params, const float* d_
 /* Strided batch of `count` equal requests: signal y is d_in + y * in_stride
  * (full_size floats), its result goes to d_out + y * out_stride (out_stride a
  * multiple of 4 floats, at least the layout's out_floats).  One launch (for
- * windows up to 2^14); results are bitwise those of ostft_run_device on each
+ * windows up to 2^15); results are bitwise those of ostft_run_device on each
  * signal.  No counterpart in the reference (one signal per ActivateSTFT). */
 int ostft_run_strided_device(ostft_ctx* ctx, const ostft_params* params, const float* d_in,
                              uint64_t full_size, uint64_t in_stride, uint32_t count, float* d_out,
@@ -168,7 +168,7 @@ typedef struct ostft_job {
 } ostft_job;
 
 /* Runs `njobs` device-resident requests of any window size, overlap, window
- * and output layout: one launch per window size present (windows 2^6..2^14)
+ * and output layout: one launch per window size present (windows 2^6..2^15)
  * over a device-side queue of tiles of frames, either one workgroup per tile
  * or a persistent grid whose workgroups pull tiles from a counter; larger
  * windows run after them, request by request.  Every

@@ -27,7 +27,7 @@ namespace ostft {
 
 namespace {
 
-constexpr int LOG_M_MIN = 5, LOG_M_MAX = 13, NCLS = LOG_M_MAX - LOG_M_MIN + 1;
+constexpr int LOG_M_MIN = 5, LOG_M_MAX = 14, NCLS = LOG_M_MAX - LOG_M_MIN + 1;
 constexpr size_t ALIGN = 256;
 
 size_t align_up(size_t x) { return (x + ALIGN - 1) & ~(ALIGN - 1); }

@@ -42,7 +42,7 @@ struct FusedArgs {
 enum Transform : int { XF_FP32 = 0, XF_MFMA_FP16 = 1, XF_MFMA_BF16 = 2 };
 bool mfma_supported(int log_m);
 
-// Single-kernel path: complex size M = 2^log_m, 5 <= log_m <= 13 (N = 64 .. 16384).
+// Single-kernel path: complex size M = 2^log_m, 5 <= log_m <= 14 (N = 64 .. 32768).
 bool fused_supported(int log_m);
 // `count` signals (grid.y), signal y at a.in + y in_stride / a.out + y out_stride
 hipError_t launch_fused(int log_m, int xf, const FusedArgs& a, uint32_t count, hipStream_t s);

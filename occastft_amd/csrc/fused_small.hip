@@ -1,5 +1,5 @@
 // fused_small.hip -- one kernel per window-size class for windows N = 2^6 ..
-// 2^14: overlap gather + window + DC removal + FFT + output epilogue, each
+// 2^15: overlap gather + window + DC removal + FFT + output epilogue, each
 // frame read from HBM once and its spectrum written once.
 //
 // What it replaces (all in one launch): Overlap_Common, Window_*,
@@ -55,7 +55,7 @@ namespace ostft {
 // spills, else 1 (the compiler's choice).
 template <int LOG_M>
 struct BatchWaves {
-    static constexpr int value = (LOG_M == 9 || LOG_M == 11 || LOG_M == 13) ? 1 : 4;
+    static constexpr int value = (LOG_M == 9 || LOG_M == 11 || LOG_M >= 13) ? 1 : 4;
 };
 
 // FusedArgs as the batch kernel reads it from the descriptor block: the same
@@ -544,7 +544,7 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
 }
 
 template <int LOG_M, int XF>
-__global__ void __launch_bounds__(Cfg<LOG_M>::WG) stft_fused_kernel(const FusedArgs a)
+__global__ void __launch_bounds__(Cfg<LOG_M>::WG) __attribute__((amdgpu_waves_per_eu(4))) stft_fused_kernel(const FusedArgs a)
 {
     using C = Cfg<LOG_M>;
     __shared__ float2 lds[C::G * C::PADM];
@@ -694,7 +694,7 @@ static const void* batch_fn_t(bool persistent)
                       : reinterpret_cast<const void*>(&stft_batch_dispatch_kernel<LOG_M>);
 }
 
-bool fused_supported(int log_m) { return log_m >= 5 && log_m <= 13; }
+bool fused_supported(int log_m) { return log_m >= 5 && log_m <= 14; }
 
 hipError_t launch_batch(int log_m, const BatchClassArgs& b, uint32_t grid, bool persistent, hipStream_t s)
 {
@@ -708,6 +708,7 @@ hipError_t launch_batch(int log_m, const BatchClassArgs& b, uint32_t grid, bool 
     case 11: return launch_batch_t<11>(b, grid, persistent, s);
     case 12: return launch_batch_t<12>(b, grid, persistent, s);
     case 13: return launch_batch_t<13>(b, grid, persistent, s);
+    case 14: return launch_batch_t<14>(b, grid, persistent, s);
     default: return hipErrorInvalidValue;
     }
 }
@@ -724,6 +725,7 @@ const void* batch_kernel_fn(int log_m, bool persistent)
     case 11: return batch_fn_t<11>(persistent);
     case 12: return batch_fn_t<12>(persistent);
     case 13: return batch_fn_t<13>(persistent);
+    case 14: return batch_fn_t<14>(persistent);
     default: return nullptr;
     }
 }
@@ -740,6 +742,7 @@ hipError_t batch_class_info(int log_m, BatchClassInfo* info)
     case 11: return batch_info_t<11>(info);
     case 12: return batch_info_t<12>(info);
     case 13: return batch_info_t<13>(info);
+    case 14: return batch_info_t<14>(info);
     default: return hipErrorInvalidValue;
     }
 }
@@ -756,6 +759,7 @@ hipError_t launch_fused(int log_m, int xf, const FusedArgs& a, uint32_t count, h
     case 11: return launch_xf<11>(xf, a, count, s);
     case 12: return launch_xf<12>(xf, a, count, s);
     case 13: return launch_xf<13>(xf, a, count, s);
+    case 14: return launch_xf<14>(xf, a, count, s);
     default: return hipErrorInvalidValue;
     }
 }

@@ -1,5 +1,6 @@
-// multipass.hip -- STFT for complex transform sizes M = 2^13 .. 2^19
-// (windows 2^14 .. 2^20).
+// multipass.hip -- STFT for complex transform sizes M = 2^15 .. 2^19
+// (windows 2^16 .. 2^20; the plan also covers M = 2^13, 2^14, which the
+// single kernel handles).
 //
 // Same pipeline as the fused kernel (gather + window, FFT, split epilogue),
 // but the transform no longer fits one workgroup's LDS, so it is cut into two
