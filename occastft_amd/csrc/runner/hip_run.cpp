@@ -19,6 +19,7 @@
 #include <sys/mman.h>
 #include <unistd.h>
 
+#include <condition_variable>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -61,14 +62,32 @@ struct ShmMap {
 // the HIP engine behind ActivateSTFT.
 class HipRunner {
 public:
-    explicit HipRunner(int port) : server_(port)
+    explicit HipRunner(int port) : server_(port) {}
+    ~HipRunner()
     {
-        InitEnv();
-        BuildKernel();
+        if (init_thread_.joinable())
+            init_thread_.join();
+        UnInit();
     }
-    ~HipRunner() { UnInit(); }
 
-    bool ServerInit(std::string& err) { return server_.listen(err); }
+    // Listens first and brings the device up on a thread of its own: the
+    // reference proxy gives a runner five one-second connection attempts
+    // (STFTProxy.cpp:136-145), less than a cold device's start-up, so the
+    // WebSocket handshake must not wait for it.  Requests that arrive earlier
+    // wait in ActivateSTFT until the environment is ready.
+    bool ServerInit(std::string& err)
+    {
+        if (!server_.listen(err))
+            return false;
+        init_thread_ = std::thread([this] {
+            InitEnv();
+            BuildKernel();
+            std::lock_guard<std::mutex> lk(ready_mu_);
+            ready_ = true;
+            ready_cv_.notify_all();
+        });
+        return true;
+    }
     void Serve()
     {
         server_.serve([this](ostft_wire::WsConnection& c, int op, std::string& msg) { OnMessage(c, op, msg); });
@@ -82,6 +101,10 @@ public:
     bool ActivateSTFT(const std::vector<float>& in, int exp, float overlap, const std::string& options,
                       std::vector<float>& out, std::string& why)
     {
+        {
+            std::unique_lock<std::mutex> lk(ready_mu_);
+            ready_cv_.wait(lk, [this] { return ready_; });
+        }
         if (!srv_) {
             why = "no HIP device";
             return false;
@@ -246,6 +269,10 @@ private:
     }
 
     ostft_wire::WsServer server_;
+    std::thread init_thread_;
+    std::mutex ready_mu_;
+    std::condition_variable ready_cv_;
+    bool ready_ = false;  // InitEnv has finished (srv_ set, or null: no device)
     ostft_server* srv_ = nullptr;
     ostft_ctx* big_ctx_ = nullptr;  // requests larger than the server's arena
     std::mutex big_mu_;
