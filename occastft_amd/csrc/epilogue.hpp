@@ -48,6 +48,24 @@ __device__ __forceinline__ BinPair finish_bin(float2 zk, float2 zm, float2 w, in
     return r;
 }
 
+// finish_bin with the stage choice fixed at compile time (no dc0 case)
+template <bool COMBINE>
+__device__ __forceinline__ BinPair finish_pair(float2 zk, float2 zm, float2 w)
+{
+    const float2 E = make_float2(zk.x + zm.x, zk.y - zm.y);  // 2 E
+    const float2 O = make_float2(zk.y + zm.y, zm.x - zk.x);  // 2 O
+    BinPair r;
+    if constexpr (COMBINE) {
+        const float2 t = cmul(w, O);
+        r.lo = cadd(E, t);
+        r.hi = csub(E, t);
+    } else {
+        r.lo = E;
+        r.hi = O;
+    }
+    return r;
+}
+
 // the output conversions of a doubled value x2 = 2 X:
 // sqrt(Re^2 + Re^2) of the Serial runner's ToPower, as sqrt(2)|Re|
 __device__ __forceinline__ float compat_power(float2 x2)

@@ -355,113 +355,127 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
     }
 
     // ---- 5. last radix-2 stage + output, the whole block's output at once ----
+    // The layout and the stage choice are uniform; each combination gets its
+    // own straight-line code (no per-value selects between layouts).  Only the
+    // reference's odd-exponent result skips the last stage, and the single
+    // kernel meets it at exponents 13 and 15 alone.
     float* __restrict__ out = a.out;
-    // mirrored blocks: X[M-k] = conj X[M+k] and X[N-k] = conj X[k] after the
-    // last radix-2 stage; for [E, O] (no last stage) E[M-k] = conj E[k] and
-    // O[M-k] = conj O[k] go to positions M-k and N-k instead
-    const uint32_t hm = a.combine ? (uint32_t)M : (uint32_t)N;  // mirror of the M+k block
-    const uint32_t lm = a.combine ? (uint32_t)N : (uint32_t)M;  // mirror of the k block
-    if (mode == OUT_HALF_COMPLEX) {
-        const uint64_t lim = a.out_limit;  // complex values
-        float4* o = reinterpret_cast<float4*>(out);
+    constexpr bool VAR_COMBINE = LOG_M == 12 || LOG_M == 14;
+    auto epilogue = [&](auto mode_c, auto comb_c) {
+        constexpr int MODE = decltype(mode_c)::value;
+        constexpr bool COMB = decltype(comb_c)::value;
+        // mirrored blocks: X[M-k] = conj X[M+k] and X[N-k] = conj X[k] after
+        // the last radix-2 stage; for [E, O] (no last stage) E[M-k] = conj E[k]
+        // and O[M-k] = conj O[k] go to positions M-k and N-k instead
+        constexpr uint32_t hm = COMB ? (uint32_t)M : (uint32_t)N;  // mirror of the M+k block
+        constexpr uint32_t lm = COMB ? (uint32_t)N : (uint32_t)M;  // mirror of the k block
+        if constexpr (MODE == OUT_HALF_COMPLEX) {
+            const uint64_t lim = a.out_limit;  // complex values
+            float4* o = reinterpret_cast<float4*>(out);
 #pragma unroll
-        for (int it = 0; it < PITER; ++it) {
-            const uint32_t i = tid + it * WG;
-            const uint32_t g = i / PPF, k0 = 2 * (i & (PPF - 1));
-            if (g >= nf)
-                break;
-            // Z[k0], Z[k0+1] and Z[M-k0-2], Z[M-k0-1] each share a row
-            // of the padded layout
-            const float2* z = lds + g * PADM;
-            const float2* zf = z + pad(k0);
-            const float2* zm = z + pad((uint32_t)M - k0 - 2);
-            const float2 w1 = cmul(eb[it], ec1), w2 = cmul(eb[it], ec2);
-            const BinPair b0 = finish_bin(zf[0], z[pad(((uint32_t)M - k0) & (M - 1))], eb[it], a.combine, false);
-            const BinPair b1 = finish_bin(zf[1], zm[1], w1, a.combine, false);
-            const BinPair b2 = finish_bin(z[pad(k0 + 2)], zm[0], w2, a.combine, false);
-            const uint64_t base = (f0 + g) * (uint64_t)N;
-            const uint64_t c0 = base + k0, c1 = base + M + k0, c2 = base + hm - k0 - 2, c3 = base + lm - k0 - 2;
-            const float2 v0 = value_of(b0.lo), v1 = value_of(b1.lo);
-            const float2 h0 = value_of(b0.hi), h1 = value_of(b1.hi);
-            const float2 mh2 = conj_value_of(b2.hi), mh1 = conj_value_of(b1.hi);
-            const float2 ml2 = conj_value_of(b2.lo), ml1 = conj_value_of(b1.lo);
-            if (c0 < lim)
-                nt_store4(make_float4(v0.x, v0.y, v1.x, v1.y), o + c0 / 2);
-            if (c1 < lim)
-                nt_store4(make_float4(h0.x, h0.y, h1.x, h1.y), o + c1 / 2);
-            if (c2 < lim)
-                nt_store4(make_float4(mh2.x, mh2.y, mh1.x, mh1.y), o + c2 / 2);
-            if (c3 < lim)
-                nt_store4(make_float4(ml2.x, ml2.y, ml1.x, ml1.y), o + c3 / 2);
-        }
-    } else if (vec_out && SYM_REAL) {
-#pragma unroll
-        for (int it = 0; it < QITER; ++it) {
-            const uint32_t i = tid + it * WG;
-            const uint32_t g = i / QPF, k0 = 4 * (i & (QPF - 1));
-            if (g >= nf)
-                break;
-            // Z[k0 .. k0+3] and Z[M-k0-4 .. M-k0-1] each share a row of
-            // the padded layout
-            const float2* z = lds + g * PADM;
-            const float2* zf = z + pad(k0);
-            const float2* zm = z + pad((uint32_t)M - k0 - 4);
-            const float2 wk[5] = {eb[it], cmul(eb[it], ec1), cmul(eb[it], ec2), cmul(eb[it], ec3),
-                                  cmul(eb[it], ec4)};
-            BinPair b[5];
-            b[0] = finish_bin(zf[0], z[pad(((uint32_t)M - k0) & (M - 1))], wk[0], a.combine, false);
-#pragma unroll
-            for (int j = 1; j < 4; ++j)
-                b[j] = finish_bin(zf[j], zm[4 - j], wk[j], a.combine, false);
-            b[4] = finish_bin(z[pad(k0 + 4)], zm[0], wk[4], a.combine, false);
-            // the value of a conjugate is the value itself
-            float lo[5], hi[5];
-#pragma unroll
-            for (int j = 0; j < 5; ++j) {
-                lo[j] = mode == OUT_COMPAT_POWER ? compat_power(b[j].lo) : magnitude(b[j].lo);
-                hi[j] = mode == OUT_COMPAT_POWER ? compat_power(b[j].hi) : magnitude(b[j].hi);
+            for (int it = 0; it < PITER; ++it) {
+                const uint32_t i = tid + it * WG;
+                const uint32_t g = i / PPF, k0 = 2 * (i & (PPF - 1));
+                if (g >= nf)
+                    break;
+                // Z[k0], Z[k0+1] and Z[M-k0-2], Z[M-k0-1] each share a row
+                // of the padded layout
+                const float2* z = lds + g * PADM;
+                const float2* zf = z + pad(k0);
+                const float2* zm = z + pad((uint32_t)M - k0 - 2);
+                const float2 w1 = cmul(eb[it], ec1), w2 = cmul(eb[it], ec2);
+                const BinPair b0 = finish_pair<COMB>(zf[0], z[pad(((uint32_t)M - k0) & (M - 1))], eb[it]);
+                const BinPair b1 = finish_pair<COMB>(zf[1], zm[1], w1);
+                const BinPair b2 = finish_pair<COMB>(z[pad(k0 + 2)], zm[0], w2);
+                const uint64_t base = (f0 + g) * (uint64_t)N;
+                const uint64_t c0 = base + k0, c1 = base + M + k0, c2 = base + hm - k0 - 2, c3 = base + lm - k0 - 2;
+                const float2 v0 = value_of(b0.lo), v1 = value_of(b1.lo);
+                const float2 h0 = value_of(b0.hi), h1 = value_of(b1.hi);
+                const float2 mh2 = conj_value_of(b2.hi), mh1 = conj_value_of(b1.hi);
+                const float2 ml2 = conj_value_of(b2.lo), ml1 = conj_value_of(b1.lo);
+                if (c0 < lim)
+                    nt_store4(make_float4(v0.x, v0.y, v1.x, v1.y), o + c0 / 2);
+                if (c1 < lim)
+                    nt_store4(make_float4(h0.x, h0.y, h1.x, h1.y), o + c1 / 2);
+                if (c2 < lim)
+                    nt_store4(make_float4(mh2.x, mh2.y, mh1.x, mh1.y), o + c2 / 2);
+                if (c3 < lim)
+                    nt_store4(make_float4(ml2.x, ml2.y, ml1.x, ml1.y), o + c3 / 2);
             }
-            float4* o = reinterpret_cast<float4*>(out + (f0 + g) * (uint64_t)N);
-            nt_store4(make_float4(lo[0], lo[1], lo[2], lo[3]), o + k0 / 4);
-            nt_store4(make_float4(hi[0], hi[1], hi[2], hi[3]), o + (M + k0) / 4);
-            nt_store4(make_float4(hi[4], hi[3], hi[2], hi[1]), o + (hm - k0 - 4) / 4);
-            nt_store4(make_float4(lo[4], lo[3], lo[2], lo[1]), o + (lm - k0 - 4) / 4);
-        }
-    } else if (vec_out) {
-        const uint32_t nbins = nf * (uint32_t)M;
+        } else if constexpr (MODE == OUT_COMPAT_POWER || MODE == OUT_MAGNITUDE) {
+            auto conv = [](float2 x2) {
+                if constexpr (MODE == OUT_COMPAT_POWER)
+                    return compat_power(x2);
+                else
+                    return magnitude(x2);
+            };
+            if constexpr (SYM_REAL) {
 #pragma unroll
-        for (int it = 0; it < OITER4; ++it) {
-            const uint32_t e = (tid + it * WG) * 4;
-            if (e >= nbins)
-                break;
-            const uint32_t g = e >> LOG_M, k0 = e & (M - 1);
-            const float2* z = lds + g * PADM;
-            const float2 wk[4] = {eb[it], cmul(eb[it], ec1), cmul(eb[it], ec2), cmul(eb[it], ec3)};
-            float lo[4], hi[4];
+                for (int it = 0; it < QITER; ++it) {
+                    const uint32_t i = tid + it * WG;
+                    const uint32_t g = i / QPF, k0 = 4 * (i & (QPF - 1));
+                    if (g >= nf)
+                        break;
+                    // Z[k0 .. k0+3] and Z[M-k0-4 .. M-k0-1] each share a row of
+                    // the padded layout
+                    const float2* z = lds + g * PADM;
+                    const float2* zf = z + pad(k0);
+                    const float2* zm = z + pad((uint32_t)M - k0 - 4);
+                    const float2 wk[5] = {eb[it], cmul(eb[it], ec1), cmul(eb[it], ec2), cmul(eb[it], ec3),
+                                          cmul(eb[it], ec4)};
+                    BinPair b[5];
+                    b[0] = finish_pair<COMB>(zf[0], z[pad(((uint32_t)M - k0) & (M - 1))], wk[0]);
 #pragma unroll
-            for (uint32_t j = 0; j < 4; ++j) {
-                const uint32_t k = k0 + j, km = (M - k) & (M - 1);
-                const BinPair b = finish_bin(z[pad(k)], z[pad(km)], wk[j], a.combine, false);
-                lo[j] = mode == OUT_COMPAT_POWER ? compat_power(b.lo) : magnitude(b.lo);
-                hi[j] = mode == OUT_COMPAT_POWER ? compat_power(b.hi) : magnitude(b.hi);
+                    for (int j = 1; j < 4; ++j)
+                        b[j] = finish_pair<COMB>(zf[j], zm[4 - j], wk[j]);
+                    b[4] = finish_pair<COMB>(z[pad(k0 + 4)], zm[0], wk[4]);
+                    // the value of a conjugate is the value itself
+                    float lo[5], hi[5];
+#pragma unroll
+                    for (int j = 0; j < 5; ++j) {
+                        lo[j] = conv(b[j].lo);
+                        hi[j] = conv(b[j].hi);
+                    }
+                    float4* o = reinterpret_cast<float4*>(out + (f0 + g) * (uint64_t)N);
+                    nt_store4(make_float4(lo[0], lo[1], lo[2], lo[3]), o + k0 / 4);
+                    nt_store4(make_float4(hi[0], hi[1], hi[2], hi[3]), o + (M + k0) / 4);
+                    nt_store4(make_float4(hi[4], hi[3], hi[2], hi[1]), o + (hm - k0 - 4) / 4);
+                    nt_store4(make_float4(lo[4], lo[3], lo[2], lo[1]), o + (lm - k0 - 4) / 4);
+                }
+            } else {
+                const uint32_t nbins = nf * (uint32_t)M;
+#pragma unroll
+                for (int it = 0; it < OITER4; ++it) {
+                    const uint32_t e = (tid + it * WG) * 4;
+                    if (e >= nbins)
+                        break;
+                    const uint32_t g = e >> LOG_M, k0 = e & (M - 1);
+                    const float2* z = lds + g * PADM;
+                    const float2 wk[4] = {eb[it], cmul(eb[it], ec1), cmul(eb[it], ec2), cmul(eb[it], ec3)};
+                    float lo[4], hi[4];
+#pragma unroll
+                    for (uint32_t j = 0; j < 4; ++j) {
+                        const uint32_t k = k0 + j, km = (M - k) & (M - 1);
+                        const BinPair b = finish_pair<COMB>(z[pad(k)], z[pad(km)], wk[j]);
+                        lo[j] = conv(b.lo);
+                        hi[j] = conv(b.hi);
+                    }
+                    float4* o = reinterpret_cast<float4*>(out + (f0 + g) * (uint64_t)N);
+                    nt_store4(make_float4(lo[0], lo[1], lo[2], lo[3]), o + k0 / 4);
+                    nt_store4(make_float4(hi[0], hi[1], hi[2], hi[3]), o + (M + k0) / 4);
+                }
             }
-            float4* o = reinterpret_cast<float4*>(out + (f0 + g) * (uint64_t)N);
-            nt_store4(make_float4(lo[0], lo[1], lo[2], lo[3]), o + k0 / 4);
-            nt_store4(make_float4(hi[0], hi[1], hi[2], hi[3]), o + (M + k0) / 4);
-        }
-    } else {
-        if constexpr (SYM_REAL) {
+        } else if constexpr (SYM_REAL) {
             // POWER / HALF_SPECTRUM: M+1 values per frame.  Slot k = 0 .. M/2 of
             // a frame gives positions k and M-k from one pair (Z[k], Z[M-k]):
-            // X[k] and X[M-k] = conj X[M+k] after the last radix-2 stage, E[k] and
-            // E[M-k] = conj E[k] for [E, O]; position M (from slot 0) is X[M] or
-            // O[0].  Consecutive slots sit on consecutive lanes, so each store
-            // instruction covers contiguous memory (ascending for k, descending
-            // for M-k).
+            // X[k] and X[M-k] = conj X[M+k] after the last radix-2 stage, E[k]
+            // and E[M-k] = conj E[k] for [E, O]; position M (from slot 0) is
+            // X[M] or O[0].  Consecutive slots sit on consecutive lanes, so each
+            // store instruction covers contiguous memory (ascending for k,
+            // descending for M-k).
             constexpr uint32_t SPF = M / 2 + 1;  // slots per frame
             const uint32_t nslot = nf * SPF;
             const uint64_t obase = f0 * (uint64_t)(M + 1);
-            const float2 one = make_float2(1.0f, 0.0f);
 #pragma unroll
             for (int it = 0; it < OITERS; ++it) {
                 const uint32_t e = tid + it * WG;
@@ -469,13 +483,12 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
                     break;
                 const uint32_t g = e / SPF, k = e - g * SPF;
                 const float2* z = lds + g * PADM;
-                const BinPair b = finish_bin(z[pad(k)], z[pad(((uint32_t)M - k) & (M - 1))], a.combine ? eb[it] : one,
-                                             a.combine, false);
-                const float2 x1 = b.lo;                                   // position k (doubled)
-                const float2 x2 = k == 0 ? b.hi : (a.combine ? b.hi : b.lo);  // position M-k, before conj
+                const BinPair b = finish_pair<COMB>(z[pad(k)], z[pad(((uint32_t)M - k) & (M - 1))], eb[it]);
+                const float2 x1 = b.lo;                            // position k (doubled)
+                const float2 x2 = (k == 0 || COMB) ? b.hi : b.lo;  // position M-k, before conj
                 const bool second = k != (uint32_t)(M / 2);
                 const uint64_t o = obase + (uint64_t)g * (M + 1);
-                if (mode == OUT_POWER) {
+                if constexpr (MODE == OUT_POWER) {
                     nt_store(power_of(x1), out + o + k);
                     if (second)
                         nt_store(power_of(x2), out + o + (M - k));
@@ -487,10 +500,11 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
                 }
             }
         } else {
-            // POWER / HALF_SPECTRUM at the smallest window: two consecutive values
-            // of the flattened [frame][bin] index per thread and iteration, one
-            // 8-byte (POWER) or 16-byte (HALF_SPECTRUM) store where aligned (the
-            // slot form's runs would be shorter than a cache line)
+            // POWER / HALF_SPECTRUM at the smallest window: two consecutive
+            // values of the flattened [frame][bin] index per thread and
+            // iteration, one 8-byte (POWER) or 16-byte (HALF_SPECTRUM) store
+            // where aligned (the slot form's runs would be shorter than a cache
+            // line)
             const uint32_t nval = nf * (uint32_t)(M + 1);
             const uint64_t obase = f0 * (uint64_t)(M + 1);
             const float2 one = make_float2(1.0f, 0.0f);
@@ -502,23 +516,23 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
                 const bool has2 = e1 + 1 < nval;
                 const uint32_t g1 = e1 / (uint32_t)(M + 1), k1 = e1 - g1 * (uint32_t)(M + 1);
                 const uint32_t kb1 = k1 == (uint32_t)M ? 0u : k1;
-                const float2 w1 = a.combine ? eb[it] : one;
+                const float2 w1 = eb[it];
                 const float2* z1 = lds + g1 * PADM;
-                const BinPair b1 = finish_bin(z1[pad(kb1)], z1[pad((M - kb1) & (M - 1))], w1, a.combine, false);
+                const BinPair b1 = finish_pair<COMB>(z1[pad(kb1)], z1[pad((M - kb1) & (M - 1))], w1);
                 const float2 x1 = k1 == (uint32_t)M ? b1.hi : b1.lo;  // doubled
                 // the next value: bin k1 + 1 of the same frame, or bin 0 of the next
                 const uint32_t g2 = k1 == (uint32_t)M ? g1 + 1 : g1, k2 = k1 == (uint32_t)M ? 0u : k1 + 1;
                 const uint32_t kb2 = k2 == (uint32_t)M ? 0u : k2;
                 float2 x2 = make_float2(0.0f, 0.0f);
                 if (has2) {
-                    const float2 w2 = (a.combine && kb2 != 0u) ? cmul(w1, ec1) : one;
+                    const float2 w2 = kb2 != 0u ? cmul(w1, ec1) : one;
                     const float2* z2 = lds + g2 * PADM;
-                    const BinPair b2 = finish_bin(z2[pad(kb2)], z2[pad((M - kb2) & (M - 1))], w2, a.combine, false);
+                    const BinPair b2 = finish_pair<COMB>(z2[pad(kb2)], z2[pad((M - kb2) & (M - 1))], w2);
                     x2 = k2 == (uint32_t)M ? b2.hi : b2.lo;
                 }
                 const uint64_t o = obase + e1;
                 const bool pair = has2 && (o & 1u) == 0;
-                if (mode == OUT_POWER) {
+                if constexpr (MODE == OUT_POWER) {
                     const float p1 = power_of(x1), p2 = power_of(x2);
                     if (pair) {
                         nt_store(make_float2(p1, p2), reinterpret_cast<float2*>(out + o));
@@ -540,6 +554,24 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
                 }
             }
         }
+    };
+    auto by_stage = [&](auto mode_c) {
+        if constexpr (VAR_COMBINE) {
+            if (a.combine)
+                epilogue(mode_c, std::true_type{});
+            else
+                epilogue(mode_c, std::false_type{});
+        } else {
+            epilogue(mode_c, std::true_type{});
+        }
+    };
+    switch (mode) {
+    case OUT_COMPAT_POWER: by_stage(std::integral_constant<int, OUT_COMPAT_POWER>{}); break;
+    case OUT_MAGNITUDE: by_stage(std::integral_constant<int, OUT_MAGNITUDE>{}); break;
+    case OUT_HALF_COMPLEX: by_stage(std::integral_constant<int, OUT_HALF_COMPLEX>{}); break;
+    case OUT_POWER: by_stage(std::integral_constant<int, OUT_POWER>{}); break;
+    case OUT_HALF_SPECTRUM: by_stage(std::integral_constant<int, OUT_HALF_SPECTRUM>{}); break;
+    default: break;
     }
 }
 
