@@ -229,12 +229,16 @@ __global__ void __launch_bounds__(MP_THREADS) last_pass_kernel(const LastArgs a)
 
 int multipass_plan(int log_m, int* lr)
 {
-    // two passes: the last radix at most 2^8 (a block and its mirror block of
-    // at least 16 butterflies share one workgroup, so output runs are >= 64 B;
-    // radix 2^9 measured 1.7x slower), the first at most 2^11
+    // two passes: the last radix small, so that a block and its mirror block
+    // (CB = 4096 / R1 butterflies each) write long runs of consecutive bins
+    // (R1 = 2^6: 256-byte runs of the real layouts; 2^8 gave 64-byte runs and
+    // a 4-5% slower large-window step, 2^9 was 1.7x slower), the first radix
+    // at most 2^11 (its column segments are CB = 8192 / R0 values)
     if (log_m < 13 || log_m > 19)
         return 0;
-    lr[1] = log_m / 2 > 8 ? 8 : log_m / 2;
+    lr[1] = log_m / 2 > 6 ? 6 : log_m / 2;
+    if (log_m - lr[1] > 11)
+        lr[1] = log_m - 11;
     lr[0] = log_m - lr[1];
     return 2;
 }

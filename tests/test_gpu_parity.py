@@ -162,6 +162,23 @@ def test_magnitude_keeps_odd_quirk(runner):
     check(y, t, N_, TOL, "magnitude quirk")
 
 
+@pytest.mark.parametrize("exp", [10, 15, 16, 17])
+def test_dc_offset_signals_against_fp64(runner, exp):
+    """A signal riding on a large offset: windowed and DC-removed, the single
+    kernel (time-domain mean) and the multi-pass path (frequency-domain k = 0
+    correction) both stay at the float32 FFT's error.  (Without a window the DC
+    bin carries the float32 rounding of the mean in any float32 pipeline, the
+    reference's included.)"""
+    N_ = 1 << exp
+    x = (1000.0 + np.random.default_rng(6000 + exp).standard_normal(5 * N_)).astype(np.float32)
+    opts = "--hanning_window --remove_dc"
+    y = runner.stft(x, exp, 0.5, opts)
+    g = ostft.geometry(ostft.parse_options(opts, exp, 0.5), len(x))
+    t = po.stft64(x, exp, g["hop"], g["frames"], po.WINDOWS["hanning"], 0.0, True, po.OUT_COMPAT_POWER,
+                  quirk_odd=True)
+    check(y, t, N_, 1e-6, f"offset 1000, exp={exp}")
+
+
 def test_exact_window_matches_fp64_on_edge_frames(runner):
     """A tail frame holding a few samples at the window's rising edge: the
     default (reference) window follows the Serial runner, --exact_window
