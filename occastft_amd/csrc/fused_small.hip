@@ -600,7 +600,7 @@ __global__ void __launch_bounds__(Cfg<LOG_M>::WG) __attribute__((amdgpu_waves_pe
 // its stores to drain.  Which form is faster for a window size is measured
 // (ostft_autotune_batch).
 template <int LOG_M>
-__global__ void __launch_bounds__(Cfg<LOG_M>::WG)
+__global__ void __launch_bounds__(Cfg<LOG_M>::WG) __attribute__((amdgpu_waves_per_eu(4)))
     stft_batch_dispatch_kernel(const char* __restrict__ blk, const uint32_t header_off)
 {
     using C = Cfg<LOG_M>;
