@@ -151,9 +151,9 @@ def test_new_output_layouts_against_fp64(runner, exp):
         check(y, t, per, 2 * TOL if mode == po.OUT_POWER else TOL, f"{flag} exp={exp}")
 
 
-def test_magnitude_keeps_odd_quirk(runner):
+@pytest.mark.parametrize("exp", [13, 15])
+def test_magnitude_keeps_odd_quirk(runner, exp):
     """--magnitude_return is the CUDA runner's |X| over the same result buffer."""
-    exp = 13
     N_ = 1 << exp
     x = np.random.default_rng(5).uniform(-1, 1, 2 * N_).astype(np.float32)
     y = runner.stft(x, exp, 0.0, "--magnitude_return")
