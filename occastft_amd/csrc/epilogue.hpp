@@ -74,11 +74,12 @@ __device__ __forceinline__ float compat_power(float2 x2)
     return HALF_SQRT2 * fabsf(x2.x);
 }
 // |X| with the hardware square root (1 ulp)
+// (fused multiply-adds written out, as in cmul)
 __device__ __forceinline__ float magnitude(float2 x2)
 {
-    return 0.5f * __builtin_amdgcn_sqrtf(x2.x * x2.x + x2.y * x2.y);
+    return 0.5f * __builtin_amdgcn_sqrtf(fmaf(x2.x, x2.x, x2.y * x2.y));
 }
-__device__ __forceinline__ float power_of(float2 x2) { return 0.25f * (x2.x * x2.x + x2.y * x2.y); }
+__device__ __forceinline__ float power_of(float2 x2) { return 0.25f * fmaf(x2.x, x2.x, x2.y * x2.y); }
 __device__ __forceinline__ float2 value_of(float2 x2) { return make_float2(0.5f * x2.x, 0.5f * x2.y); }
 __device__ __forceinline__ float2 conj_value_of(float2 x2) { return make_float2(0.5f * x2.x, -0.5f * x2.y); }
 

@@ -13,9 +13,14 @@ namespace ostft {
 
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+// The fused multiply-adds are written out, so that every kernel built from
+// this code rounds a product the same way: left to the compiler, which of
+// the two products goes into the FMA may differ from kernel to kernel (and
+// with it the last bit), and frames must be bitwise the same whichever kernel
+// ran them.
 __device__ __forceinline__ float2 cmul(float2 a, float2 b)
 {
-    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+    return make_float2(fmaf(a.x, b.x, -(a.y * b.y)), fmaf(a.x, b.y, a.y * b.x));
 }
 
 typedef float f32x2 __attribute__((ext_vector_type(2)));

@@ -576,7 +576,8 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
 }
 
 template <int LOG_M, int XF>
-__global__ void __launch_bounds__(Cfg<LOG_M>::WG) __attribute__((amdgpu_waves_per_eu(4))) stft_fused_kernel(const FusedArgs a)
+__global__ void __launch_bounds__(Cfg<LOG_M>::WG) __attribute__((amdgpu_waves_per_eu(4)))
+    stft_fused_kernel(const FusedArgs a)
 {
     using C = Cfg<LOG_M>;
     __shared__ float2 lds[C::G * C::PADM];
