@@ -17,7 +17,8 @@
 //     per 16).  The last pass leaves Z in LDS in natural order; the epilogue
 //     reads Z[k] and Z[M-k], forms the even/odd half spectra, applies the last
 //     radix-2 stage (or stops before it for the reference's odd-exponent
-//     result) and writes positions k and M+k of the frame.
+//     result) and writes positions k and M+k of the frame -- and, by the
+//     conjugate symmetry of a real frame's spectrum, positions M-k and N-k.
 //   * Frames never share a transform, so a frame's result is bitwise the same
 //     whatever else is in the launch.
 //   * A workgroup of max(256, T) threads owns G = WG/T consecutive frames.
