@@ -194,6 +194,14 @@ int ostft_batch_config(ostft_ctx* ctx, int32_t exp, int32_t blocks_per_cu, int32
 int ostft_autotune_batch(ostft_ctx* ctx, int32_t exp, uint64_t samples, int32_t* best_blocks_per_cu,
                          double* best_ms);
 
+/* Scratch budget of the two-pass path (windows 2^16..2^20), in bytes: its
+ * frames go through both passes in chunks whose intermediate spectra fit the
+ * budget (at least one frame per chunk).  bytes = 0 leaves it; a budget below
+ * the scratch already held frees that scratch.  *current (may be NULL)
+ * receives the budget in force.  Default 1 GiB.  No counterpart in the
+ * reference, whose runners hold whole-signal intermediate buffers. */
+int ostft_scratch_budget(ostft_ctx* ctx, uint64_t bytes, uint64_t* current);
+
 /* ---- serving: continuous batching of host-buffer requests ----
  *
  * No counterpart in the reference, whose runner serves each WebSocket request

@@ -55,6 +55,7 @@ EXPORTS = (
     "ostft_activate_geometry",
     "ostft_batch_config",
     "ostft_autotune_batch",
+    "ostft_scratch_budget",
     "ostft_server_create",
     "ostft_server_destroy",
     "ostft_server_submit",
@@ -187,6 +188,8 @@ def lib() -> ctypes.CDLL:
     L.ostft_batch_config.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, i32p]
     L.ostft_autotune_batch.restype = ctypes.c_int
     L.ostft_autotune_batch.argtypes = [vp, ctypes.c_int32, ctypes.c_uint64, i32p, ctypes.POINTER(ctypes.c_double)]
+    L.ostft_scratch_budget.restype = ctypes.c_int
+    L.ostft_scratch_budget.argtypes = [vp, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)]
     L.ostft_server_create.restype = ctypes.c_int
     L.ostft_server_create.argtypes = [ctypes.c_int32, ctypes.POINTER(ServerConfig), ctypes.POINTER(vp)]
     L.ostft_server_destroy.restype = None

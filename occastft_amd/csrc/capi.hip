@@ -607,6 +607,23 @@ int ostft_autotune_batch(ostft_ctx* ctx, int32_t exp, uint64_t samples, int32_t*
     return ostft::autotune_batch(ctx, exp, samples, best_blocks_per_cu, best_ms);
 }
 
+int ostft_scratch_budget(ostft_ctx* ctx, uint64_t bytes, uint64_t* current)
+{
+    if (!ctx)
+        return OSTFT_ERR_INVALID_ARG;
+    if (bytes) {
+        DeviceGuard dg(ctx->device);
+        if (!dg.ok)
+            return fail(ctx, OSTFT_ERR_NO_DEVICE, "hipSetDevice failed");
+        ctx->scratch.budget = bytes;
+        if (ctx->scratch.cap * sizeof(float2) > bytes)
+            ctx->scratch.release();
+    }
+    if (current)
+        *current = ctx->scratch.budget;
+    return OSTFT_OK;
+}
+
 namespace {
 int activate_host(ostft_ctx* ctx, const ostft_params& p, const float* in, uint64_t full_size, float* out,
                   uint64_t out_cap, uint64_t* out_len, uint64_t hop, uint64_t frames);

@@ -7,11 +7,12 @@ struct ostft_ctx;
 
 namespace ostft {
 
-// Two ping-pong complex buffers owned by a context, grown on demand.
+// The complex buffer between the two passes, owned by a context, grown on
+// demand up to one chunk of frames.
 struct Scratch {
     float2* a = nullptr;
-    float2* b = nullptr;
-    uint64_t cap = 0;  // complex elements in each buffer
+    uint64_t cap = 0;                      // complex elements
+    uint64_t budget = (uint64_t)1 << 30;  // bytes per chunk of frames
     hipError_t ensure(uint64_t n);
     void release();
 };
@@ -22,6 +23,10 @@ Scratch& ctx_scratch(ostft_ctx* ctx);
 
 // Runs the two passes (the split epilogue inside the last).  Returns an OSTFT_* status.
 int run_multipass(ostft_ctx* ctx, int log_m, const FusedArgs& a, hipStream_t s);
+
+// Frames per chunk (both passes run a chunk before the next one starts):
+// as many as fit `budget` bytes of scratch, at least one.
+uint64_t multipass_chunk_frames(uint64_t budget, int log_m);
 
 // Radix split of 2^log_m into the two passes (returns 2; 0 outside 13..19).
 int multipass_plan(int log_m, int* log_radix /* [4] */);

@@ -199,6 +199,15 @@ class Runner:
             raise STFTError(rc, self.last_error)
         return cur.value
 
+    def scratch_budget(self, nbytes: int = 0) -> int:
+        """Scratch budget (bytes) of the two-pass path for windows 2^16..2^20;
+        0 leaves it.  Returns the budget in force."""
+        cur = ctypes.c_uint64()
+        rc = N.lib().ostft_scratch_budget(self._h, int(nbytes), ctypes.byref(cur))
+        if rc != N.OK:
+            raise STFTError(rc, self.last_error)
+        return cur.value
+
     def autotune_batch(self, exp: int, samples: int = 1 << 24):
         """-> (best setting, best ms) for window exponent `exp`: 0 = one
         workgroup per tile, k > 0 = persistent with k workgroups per CU."""

@@ -118,6 +118,38 @@ def test_large_windows_against_fp64(runner, exp):
     check(y, r, N_, TOL, f"restatement exp={exp}")
 
 
+def test_multipass_long_signal_every_frame(runner):
+    """A 2^16 window at 75% overlap over 700 frames (180 MB of spectra): every
+    frame against the FP64 pipeline; then with the two-pass path's scratch
+    budget cut to 128 frames (32 MiB, six chunks) and to one frame per chunk,
+    every frame bitwise equal to the one-chunk result."""
+    exp, ov, opts = 16, 0.75, "--blackman_window --remove_dc --power_return"
+    N_ = 1 << exp
+    hop = N_ // 4
+    frames = 700
+    L = (frames - 1) * hop + N_
+    rng = np.random.default_rng(4242)
+    x = (rng.standard_normal(L) + 0.5 * np.cos(np.pi * np.arange(L) ** 2 / (2 * L))).astype(np.float32)
+    g = ostft.geometry(ostft.parse_options(opts, exp, ov), L)
+    assert g["frames"] >= frames
+    per = N_ // 2 + 1
+    default = runner.scratch_budget()
+    assert default >= g["frames"] * N_ * 4  # the whole request in one chunk
+    y = runner.stft(x, exp, ov, opts).reshape(-1, per)
+    t = po.stft64(x, exp, hop, g["frames"], po.WINDOWS["blackman"], 0.0, True, po.OUT_POWER).reshape(-1, per)
+    energy = np.linalg.norm(t, axis=1)
+    keep = energy > 1e-3 * energy.max()  # frames not dominated by the window edge
+    assert keep.sum() >= frames - 4
+    check(y[keep], t[keep], per, TOL, "700 frames")
+    try:
+        for budget in (128 * N_ * 4, 1):
+            assert runner.scratch_budget(budget) == budget
+            yc = runner.stft(x, exp, ov, opts).reshape(-1, per)
+            assert np.array_equal(yc.view(np.uint32), y.view(np.uint32)), budget
+    finally:
+        runner.scratch_budget(default)
+
+
 @pytest.mark.parametrize("exp", [13, 15])
 def test_odd_exponent_result_buffer(runner, exp):
     """Odd exponents >= 13 return the penultimate radix-2 stage (the reference's
