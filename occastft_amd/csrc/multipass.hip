@@ -26,6 +26,7 @@
 // in the frequency domain: subtracting the frame mean only changes the k = 0
 // terms.
 #include "../../include/occastft_hip.h"
+#include "batch.hpp"
 #include "epilogue.hpp"
 #include "multipass.hpp"
 #include "stockham.hpp"
@@ -343,7 +344,7 @@ int run_multipass(ostft_ctx* ctx, int log_m, const FusedArgs& a, hipStream_t s)
     const uint64_t chunk = multipass_chunk_frames(sc.budget, log_m);
     hipError_t e = sc.ensure((a.frames < chunk ? a.frames : chunk) * M);
     if (e != hipSuccess)
-        return OSTFT_ERR_HIP;
+        return ctx_hip_fail(ctx, e, "multi-pass scratch allocation");
     int rc = OSTFT_OK;
     const uint32_t fb = (uint32_t)(log_m + 1) / 2;
     const float2* tw_fine = ctx_twiddles(ctx, (int64_t)M, (int64_t)1 << fb, &rc);
@@ -396,8 +397,11 @@ int run_multipass(ostft_ctx* ctx, int log_m, const FusedArgs& a, hipStream_t s)
         const uint64_t f1 = a.frames - f0 < chunk ? a.frames : f0 + chunk;
         p.frame0 = la.frame0 = f0;
         p.frames = la.frames = f1;
-        if (first_pass(lr[0], p, s) != hipSuccess || last_pass(lr[1], la, s) != hipSuccess)
-            return OSTFT_ERR_HIP;
+        e = first_pass(lr[0], p, s);
+        if (e == hipSuccess)
+            e = last_pass(lr[1], la, s);
+        if (e != hipSuccess)
+            return ctx_hip_fail(ctx, e, "multi-pass launch");
     }
     return OSTFT_OK;
 }
