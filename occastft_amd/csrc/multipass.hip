@@ -57,6 +57,14 @@ void Scratch::release()
     cap = 0;
 }
 
+// Extra complex values per LDS column: the workgroup's global <-> LDS
+// transposition writes one value per column across consecutive lanes, so an
+// odd column stride spreads those lanes over all banks (R + R/16 is a
+// multiple of 4: up to 16 lanes of a wave on one bank).
+#ifndef OSTFT_MP_COL_PAD
+#define OSTFT_MP_COL_PAD 1
+#endif
+
 constexpr int MP_THREADS = 512;
 constexpr int MP_ELEMS = 16 * MP_THREADS;  // complex values per workgroup
 
@@ -73,7 +81,7 @@ struct PassArgs {
 template <int LOG_R>
 __global__ void __launch_bounds__(MP_THREADS) first_pass_kernel(const PassArgs a)
 {
-    constexpr int R = 1 << LOG_R, CB = MP_ELEMS / R, T = R / 16, PADR = R + R / 16;
+    constexpr int R = 1 << LOG_R, CB = MP_ELEMS / R, T = R / 16, PADR = R + R / 16 + OSTFT_MP_COL_PAD;
     static_assert(CB * T == MP_THREADS, "MP_THREADS threads per block");
     __shared__ float2 lds[CB * PADR];
 
@@ -160,7 +168,7 @@ struct LastArgs {
 template <int LOG_R>
 __global__ void __launch_bounds__(MP_THREADS) last_pass_kernel(const LastArgs a)
 {
-    constexpr int R = 1 << LOG_R, CB2 = MP_ELEMS / R, CB = CB2 / 2, T = R / 16, PADR = R + R / 16;
+    constexpr int R = 1 << LOG_R, CB2 = MP_ELEMS / R, CB = CB2 / 2, T = R / 16, PADR = R + R / 16 + OSTFT_MP_COL_PAD;
     static_assert(CB2 * T == MP_THREADS && CB >= 1, "MP_THREADS threads per block");
     __shared__ float2 lds[CB2 * PADR];
 
