@@ -133,7 +133,10 @@ void ostft_destroy(ostft_ctx* ctx);
  * aligned) are device pointers on the context's device; work is queued on `hip_stream`
  * (a hipStream_t, or NULL for the null stream) and this call does not wait
  * for it.  The first call for a new (window, exp) builds the window and
- * FP64-generated twiddle tables synchronously.  Replaces the device part of Runner::ActivateSTFT. */
+ * FP64-generated twiddle tables synchronously.  Windows 2^16..2^20 pass
+ * through the context's scratch buffer: a call on another stream than the
+ * previous such call first makes its stream wait for that call's launches.
+ * Replaces the device part of Runner::ActivateSTFT. */
 int ostft_run_device(ostft_ctx* ctx, const ostft_params* params, const float* d_in,
                      uint64_t full_size, float* d_out, uint64_t out_cap, void* hip_stream);
 

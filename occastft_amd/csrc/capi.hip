@@ -485,7 +485,7 @@ void ostft_destroy(ostft_ctx* ctx)
             (void)hipFree(ctx->d_in);
         if (ctx->d_out)
             (void)hipFree(ctx->d_out);
-        ctx->scratch.release();
+        ctx->scratch.destroy();
         ostft::batch_release(ctx);
         if (ctx->stream)
             (void)hipStreamDestroy(ctx->stream);

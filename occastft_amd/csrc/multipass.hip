@@ -47,6 +47,28 @@ hipError_t Scratch::ensure(uint64_t n)
     return hipSuccess;
 }
 
+hipError_t Scratch::acquire(hipStream_t s)
+{
+    if (!done) {
+        const hipError_t e = hipEventCreateWithFlags(&done, hipEventDisableTiming);
+        if (e != hipSuccess) {
+            done = nullptr;
+            return e;
+        }
+    }
+    return used && last != s ? hipStreamWaitEvent(s, done, 0) : hipSuccess;
+}
+
+hipError_t Scratch::handoff(hipStream_t s)
+{
+    const hipError_t e = hipEventRecord(done, s);
+    if (e == hipSuccess) {
+        last = s;
+        used = true;
+    }
+    return e;
+}
+
 void Scratch::release()
 {
     if (a) {
@@ -55,6 +77,15 @@ void Scratch::release()
     }
     a = nullptr;
     cap = 0;
+}
+
+void Scratch::destroy()
+{
+    release();
+    if (done)
+        (void)hipEventDestroy(done);
+    done = nullptr;
+    used = false;
 }
 
 // Extra complex values per LDS column: the workgroup's global <-> LDS
@@ -353,6 +384,9 @@ int run_multipass(ostft_ctx* ctx, int log_m, const FusedArgs& a, hipStream_t s)
     hipError_t e = sc.ensure((a.frames < chunk ? a.frames : chunk) * M);
     if (e != hipSuccess)
         return ctx_hip_fail(ctx, e, "multi-pass scratch allocation");
+    e = sc.acquire(s);
+    if (e != hipSuccess)
+        return ctx_hip_fail(ctx, e, "multi-pass scratch ordering");
     int rc = OSTFT_OK;
     const uint32_t fb = (uint32_t)(log_m + 1) / 2;
     const float2* tw_fine = ctx_twiddles(ctx, (int64_t)M, (int64_t)1 << fb, &rc);
@@ -411,7 +445,8 @@ int run_multipass(ostft_ctx* ctx, int log_m, const FusedArgs& a, hipStream_t s)
         if (e != hipSuccess)
             return ctx_hip_fail(ctx, e, "multi-pass launch");
     }
-    return OSTFT_OK;
+    e = sc.handoff(s);
+    return e == hipSuccess ? OSTFT_OK : ctx_hip_fail(ctx, e, "multi-pass scratch event");
 }
 
 }  // namespace ostft

@@ -8,13 +8,21 @@ struct ostft_ctx;
 namespace ostft {
 
 // The complex buffer between the two passes, owned by a context, grown on
-// demand up to one chunk of frames.
+// demand up to one chunk of frames.  Calls on different streams take turns:
+// each call's launches wait for the previous call's (an event recorded on
+// its stream) when the stream changes.
 struct Scratch {
     float2* a = nullptr;
     uint64_t cap = 0;                      // complex elements
     uint64_t budget = (uint64_t)1 << 30;  // bytes per chunk of frames
+    hipEvent_t done = nullptr;             // recorded after the last call's launches
+    hipStream_t last = nullptr;            // the stream of that call
+    bool used = false;
     hipError_t ensure(uint64_t n);
-    void release();
+    hipError_t acquire(hipStream_t s);  // order stream s after the previous user
+    hipError_t handoff(hipStream_t s);  // record the end of this call's launches
+    void release();                     // frees the buffer (waits for the device)
+    void destroy();                     // release + the event
 };
 
 // accessors into the context (defined in capi.hip)

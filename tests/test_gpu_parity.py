@@ -274,6 +274,26 @@ def test_run_device_matches_host_path(runner):
     assert np.array_equal(out.cpu().numpy().view(np.uint32), host.view(np.uint32))
 
 
+def test_two_pass_requests_on_two_streams(runner):
+    """Windows 2^16..2^20 go through the context's one scratch buffer: four
+    requests queued alternately on two streams, with no wait between them,
+    give bitwise the results each gives alone."""
+    import torch
+
+    rng = np.random.default_rng(77)
+    opts = "--blackman_window --remove_dc --power_return"
+    xs = [rng.standard_normal(40 * 65536 + 123 * i).astype(np.float32) for i in range(4)]
+    alone = [runner.stft(x, 16, 0.5, opts) for x in xs]
+    p = ostft.parse_options(opts, 16, 0.5)
+    xd = [torch.from_numpy(x).cuda() for x in xs]
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    torch.cuda.synchronize()
+    outs = [runner.run_device(x, p, stream=streams[i % 2]) for i, x in enumerate(xd)]
+    torch.cuda.synchronize()
+    for i, (o, a) in enumerate(zip(outs, alone)):
+        assert np.array_equal(o.cpu().numpy().view(np.uint32), a.view(np.uint32)), i
+
+
 @pytest.mark.parametrize("exp", [8, 10, 14, 16])
 def test_frames_are_independent_of_the_batch(runner, exp):
     """A frame's result does not depend on which other frames share the launch
