@@ -99,6 +99,12 @@ __device__ __forceinline__ FusedArgs from_global(const FusedArgsGlobal& g)
     return a;
 }
 
+// complex transform sizes from which the frame mean is removed in the
+// frequency domain (below: in the time domain)
+#ifndef OSTFT_FREQ_DC_LOG_M
+#define OSTFT_FREQ_DC_LOG_M 11
+#endif
+
 // LDS float index of staged sample q: 4 pad floats after every 128, which
 // keeps 16-byte rows aligned for the vector writes.
 __device__ __forceinline__ uint32_t spad(uint32_t q) { return q + ((q >> 7) << 2); }
@@ -331,7 +337,11 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
     }
 
     // ---- 3. DC removal of the windowed frame (DCRemove_Common) ----
-    if (a.dc) {
+    // Windows below 2^OSTFT_FREQ_DC_LOG_M+1 subtract the frame mean here (sum
+    // by in-row DPP butterflies); larger ones remove it after the FFT, where
+    // subtracting the mean from every sample only changes bin 0.
+    constexpr bool FREQ_DC = LOG_M >= OSTFT_FREQ_DC_LOG_M;
+    if (!FREQ_DC && a.dc) {
         float2 s = make_float2(0.0f, 0.0f);
 #pragma unroll
         for (int m = 0; m < 16; ++m)
@@ -346,6 +356,18 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
     // ---- 4. FFT of the packed frame; Z stays in LDS, natural order ----
     if constexpr (XF == XF_FP32) {
         stockham<LOG_M, 0>(v, lds + grp * PADM, ptw, t);
+        if constexpr (FREQ_DC) {
+            // z[n] - mu (1 + i) for all n, mu = (Re Z0 + Im Z0) / N, moves
+            // Z0 to ((Re Z0 - Im Z0) / 2, (Im Z0 - Re Z0) / 2)
+            if (a.dc) {
+                float2* z = lds + grp * PADM;
+                if (t == 0 && active) {
+                    const float2 z0 = z[0];
+                    z[0] = make_float2(0.5f * (z0.x - z0.y), 0.5f * (z0.y - z0.x));
+                }
+                __syncthreads();
+            }
+        }
     } else {
         float2* z = lds + grp * PADM;
 #pragma unroll
