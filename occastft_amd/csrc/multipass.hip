@@ -380,7 +380,11 @@ int run_multipass(ostft_ctx* ctx, int log_m, const FusedArgs& a, hipStream_t s)
         return OSTFT_ERR_UNSUPPORTED;
     const uint64_t M = 1ull << log_m;
     Scratch& sc = ctx_scratch(ctx);
-    const uint64_t chunk = multipass_chunk_frames(sc.budget, log_m);
+    // the fewest chunks the budget allows, of equal size (a request of
+    // 2^k + 1 frames gets no one-frame chunk at its end)
+    const uint64_t cap = multipass_chunk_frames(sc.budget, log_m);
+    const uint64_t nchunks = (a.frames + cap - 1) / cap;
+    const uint64_t chunk = nchunks ? (a.frames + nchunks - 1) / nchunks : 1;
     hipError_t e = sc.ensure((a.frames < chunk ? a.frames : chunk) * M);
     if (e != hipSuccess)
         return ctx_hip_fail(ctx, e, "multi-pass scratch allocation");
