@@ -143,6 +143,16 @@ def test_parameter_validation():
         ostft.geometry(ostft.make_params(10, 0.0, window=9), 1000)
 
 
+def test_context_calls_reject_a_null_context():
+    import ctypes
+
+    cur = ctypes.c_uint64(7)
+    assert N.lib().ostft_scratch_budget(None, 1 << 20, ctypes.byref(cur)) == N.ERR_INVALID_ARG
+    assert cur.value == 7
+    best = ctypes.c_int32()
+    assert N.lib().ostft_batch_config(None, 10, 1, -1, ctypes.byref(best)) == N.ERR_INVALID_ARG
+
+
 def test_create_without_gpu_is_a_clean_error():
     import torch
 
