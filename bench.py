@@ -211,13 +211,15 @@ def roofline(achieved_gbps, kernel, bytes_per_launch, launch_ms, traffic=None, *
 
 def cold_time(fn, stream, reps):
     """Mean ms of fn() between HIP events on `stream`, Infinity Cache evicted
-    (a 1 GiB buffer overwritten) before each call."""
+    before each call by reading a 1 GiB buffer: a read leaves only clean lines
+    behind, whereas overwriting a buffer would leave up to 256 MB of dirty
+    lines whose write-back then lands inside the timed launch."""
     import torch
 
-    flush = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
+    flush = torch.zeros(1 << 28, dtype=torch.float32, device="cuda")
     ts = []
     for _ in range(reps):
-        flush.zero_()
+        flush.sum()
         s0, s1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         s0.record(stream)
         fn()
@@ -740,21 +742,10 @@ def main() -> int:
     dom = int(np.argmax(per_ms))
     e, ov, name, p, g = reqs[dom]
     b = algorithmic_bytes(g, 1 << e)
-    # Roofline of the dominant kernel measured cold: before each launch a 1 GiB
-    # buffer is overwritten so that no input stays in the 256 MB Infinity Cache
-    # from the previous request; only the kernel is between the events.
-    flush = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
-    cold = []
-    for _ in range(max(3, args.steps)):
-        flush.zero_()
-        s0, s1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        s0.record(stream)
-        runner.run_device(x, p, out=out, stream=stream)
-        s1.record(stream)
-        torch.cuda.synchronize()
-        cold.append(s0.elapsed_time(s1))
-    del flush
-    cold_ms = float(np.mean(cold))
+    # Roofline of the dominant kernel measured cold: no input of the previous
+    # request stays in the 256 MB Infinity Cache; only the kernel is between
+    # the events.
+    cold_ms = cold_time(lambda: runner.run_device(x, p, out=out, stream=stream), stream, max(3, args.steps))
     achieved = b / (cold_ms / 1e3) / 1e9
     total_bytes = sum(algorithmic_bytes(g, 1 << e) for e, ov, name, p, g in reqs)
     kernel_key = f"stft_fused_kernel<{e - 1}, 0> window {1 << e} overlap {ov} {name}"
