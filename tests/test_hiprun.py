@@ -188,6 +188,22 @@ def test_request_through_capnp_list(runner, far):
 
 
 @pytest.mark.gpu
+def test_large_window_request(runner):
+    """A 2^17 window (the two-pass path behind the server) through the wire
+    protocol, against the float restatement with a 64-bit twiddle index."""
+    exp, ov, opts = 17, 0.5, "--blackman_window --remove_dc"
+    x = np.random.default_rng(3).standard_normal(3 * (1 << exp) + 77).astype(np.float32)
+    c = runner.client()
+    c.send_binary(wc.encode(wc.Req(mappedID="L", windowRadix=exp, overlapRatio=ov, dataLength=len(x),
+                                   options=opts, data=x)))
+    op, p = c.recv_message()
+    rep = wc.decode(p)
+    assert rep.sharedMemory == "" and rep.mappedID == "L"
+    assert rep.data.size == po.geometry(len(x), exp, ov)["ofull"]
+    _check(rep.data, x, exp, ov, opts)
+
+
+@pytest.mark.gpu
 def test_request_through_shared_memory(runner):
     x = np.random.default_rng(2).uniform(-1, 1, 300_000).astype(np.float32)
     opts = "--blackman_window --remove_dc --half_complex_return"
