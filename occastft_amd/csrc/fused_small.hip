@@ -337,10 +337,11 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
     }
 
     // ---- 3. DC removal of the windowed frame (DCRemove_Common) ----
-    // Windows below 2^OSTFT_FREQ_DC_LOG_M+1 subtract the frame mean here (sum
-    // by in-row DPP butterflies); larger ones remove it after the FFT, where
-    // subtracting the mean from every sample only changes bin 0.
-    constexpr bool FREQ_DC = LOG_M >= OSTFT_FREQ_DC_LOG_M;
+    // Windows below 2^OSTFT_FREQ_DC_LOG_M+1 (and the matrix-core forms)
+    // subtract the frame mean here (sum by in-row DPP butterflies); larger
+    // ones remove it after the FFT, where subtracting the mean from every
+    // sample only changes bin 0.
+    constexpr bool FREQ_DC = XF == XF_FP32 && LOG_M >= OSTFT_FREQ_DC_LOG_M;
     if (!FREQ_DC && a.dc) {
         float2 s = make_float2(0.0f, 0.0f);
 #pragma unroll
