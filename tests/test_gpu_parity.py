@@ -353,6 +353,54 @@ def test_config2_sampled_frames_full_size(runner):
                 del out
 
 
+def test_config3_sampled_frames_full_size(runner):
+    """2^27 samples of N(0,1) noise + a linear chirp, windows 2^16..2^20,
+    overlap 0 / 75%, Blackman + DC, power spectrum, with the default scratch
+    budget (the 75% requests run in three chunks of frames): the first and
+    last frames and those either side of every third of the request against
+    the FP64 pipeline run on each frame's own span."""
+    import torch
+
+    L = 1 << 27
+    rng = np.random.default_rng(0x5EED0003)
+    xd = torch.empty(L, dtype=torch.float32, device="cuda")
+    gen = torch.Generator(device="cuda").manual_seed(3)
+    for s in range(0, L, 1 << 24):
+        n = torch.arange(s, s + (1 << 24), dtype=torch.int64, device="cuda")
+        ph = ((n * n) % (4 * L)).double() * (np.pi / (2 * L))
+        xd[s: s + (1 << 24)] = (torch.randn(n.shape, dtype=torch.float64, device="cuda", generator=gen)
+                                + 0.5 * torch.cos(ph)).float()
+    del n, ph
+    opts = "--blackman_window --remove_dc --power_return"
+    for exp in range(16, 21):
+        N_ = 1 << exp
+        per = N_ // 2 + 1
+        for ov in (0.0, 0.75):
+            p = ostft.parse_options(opts, exp, ov)
+            g = ostft.geometry(p, L)
+            out = runner.run_device(xd, p)
+            torch.cuda.synchronize()
+            frames, hop = g["frames"], g["hop"]
+            thirds = [frames // 3, 2 * frames // 3]
+            pick = sorted({0, 1, frames - 2, frames - 1, *[t + d for t in thirds for d in (-1, 0, 1)],
+                           *rng.integers(0, frames, 4).tolist()})
+            pairs = []
+            for f in pick:
+                s0 = f * hop
+                seg = np.zeros(N_, np.float32)
+                part = xd[s0: s0 + N_].cpu().numpy()
+                seg[: part.size] = part
+                t = po.stft64(seg, exp, hop, 1, po.WINDOWS["blackman"], 0.0, True, po.OUT_POWER)
+                pairs.append((f, out[f * per: (f + 1) * per].cpu().numpy(), t))
+            top = max(np.linalg.norm(t) for _, _, t in pairs)
+            for f, got, t in pairs:
+                if np.linalg.norm(t) > 1e-3 * top:  # frames not dominated by the window edge
+                    check(got, t, per, TOL, f"exp={exp} ov={ov} f={f}")
+                elif f * hop >= L:
+                    assert not np.any(got), f  # frames past the signal are zero
+            del out
+
+
 def spectrum_of(seg, exp, opts):
     """Re/Im of the first frame of `seg` from the restated Serial half-complex
     output (two frames' worth: the buffer half holds frame 0 whole)."""
