@@ -148,3 +148,35 @@ def test_strided_batch_matches_single_requests(runner, flags):
             one = runner.run_device(x[y].contiguous(), p)
             torch.cuda.synchronize()
             assert np.array_equal(out[y].cpu().numpy().view(np.uint32), one.cpu().numpy().view(np.uint32))
+
+
+def test_config4_full_size(runner):
+    """4096 signals x 2^18 samples of unit-variance 1/f noise, windows 64 / 256
+    / 1024, 50%, Hann, half spectrum, one strided launch per form: every frame
+    of four signals against the FP64 pipeline (FP32 and FP16 forms <= 1e-5,
+    BF16 <= 1e-4)."""
+    import torch
+
+    S, L = 4096, 1 << 18
+    x = torch.empty(S, L, dtype=torch.float32, device="cuda")
+    gen = torch.Generator(device="cuda").manual_seed(4)
+    f = torch.arange(L // 2 + 1, dtype=torch.float32, device="cuda")
+    shape = torch.where(f > 0, 1.0 / torch.sqrt(torch.clamp(f, min=1.0)), torch.zeros_like(f))
+    for s in range(0, S, 256):
+        w = torch.randn(256, L, dtype=torch.float32, device="cuda", generator=gen)
+        y = torch.fft.irfft(torch.fft.rfft(w, dim=1) * shape, n=L, dim=1)
+        x[s: s + 256] = y / y.std(dim=1, keepdim=True)
+    del w, y
+    rows = (0, 1, 2047, 4095)
+    xs = {r: x[r].cpu().numpy() for r in rows}
+    for exp in EXPS:
+        for flags, tol in ((0, 1e-5), (N.FLAG_MFMA_FP16, 1e-5), (N.FLAG_MFMA_BF16, 1e-4)):
+            p = ostft.make_params(exp, 0.5, N.WIN_HANNING, 0.0, N.OUT_HALF_SPECTRUM, EXACT | flags)
+            out = runner.run_strided_device(x, p)
+            torch.cuda.synchronize()
+            for r in rows:
+                ref, g = fp64(xs[r], p)
+                got = out[r, : g["out_floats"]].cpu().numpy()
+                mx, _, zabs = frame_errors(got, ref, (1 << exp) + 2)
+                assert mx <= tol and zabs <= 1e-6, (exp, flags, r, mx)
+            del out
