@@ -231,6 +231,41 @@ def test_server_results_alone_and_in_the_mix(runner):
     assert 0 < st["latency_p50_ms"] <= st["latency_p99_ms"] <= st["latency_max_ms"]
 
 
+def test_server_full_request_distribution(runner):
+    """200 requests of the serving workload's distribution (lengths
+    log-uniform 2^14..2^22, windows 2^8..2^13, overlap 0/25/50/75%, Hann /
+    Hamming / Blackman, power spectrum) submitted at once from four threads
+    to a server with the default arena: every result bitwise that of
+    ostft_run_device alone."""
+    rng = np.random.default_rng(5)
+    wins = (N.WIN_HANNING, N.WIN_HAMMING, N.WIN_BLACKMAN)
+    reqs = []
+    for _ in range(200):
+        L = int(np.exp(rng.uniform(np.log(2.0 ** 14), np.log(2.0 ** 22))))
+        p = ostft.make_params(int(rng.integers(8, 14)), float(rng.choice([0.0, 0.25, 0.5, 0.75])),
+                              int(rng.choice(wins)), 0.0, N.OUT_POWER, N.FLAG_REMOVE_DC)
+        f, a = rng.uniform(0, 0.5), rng.uniform(0.1, 1.0)
+        x = (rng.uniform(-1, 1, L) + a * np.sin(2 * np.pi * f * np.arange(L))).astype(np.float32)
+        reqs.append((x, p))
+    alone = [run_alone(runner, x, p) for x, p in reqs]
+    results = {}
+    with ostft.Server(0) as srv:
+        def client(idx):
+            tickets = [(i, srv.submit(reqs[i][0], reqs[i][1])) for i in idx]
+            for i, t in tickets:
+                results[i] = srv.wait(t)
+
+        ts = [threading.Thread(target=client, args=(list(range(k, len(reqs), 4)),)) for k in range(4)]
+        for t in ts:
+            t.start()
+        for t in ts:
+            t.join(timeout=120)
+        st = srv.stats()
+    assert len(results) == len(reqs) and st["failed"] == 0
+    for i in range(len(reqs)):
+        assert bitwise_equal(results[i], alone[i]), i
+
+
 def test_server_admission_by_footprint(runner):
     """A small arena: requests wait for memory to free up and still all
     complete correctly; a request larger than the arena is refused at submit."""
