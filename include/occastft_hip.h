@@ -16,9 +16,18 @@
  * Conventions: plain C types only; 0 = OSTFT_OK, other values are the
  * OSTFT_ERR_* codes below; no exception crosses this boundary.  The caller
  * owns input and output memory; the context owns device tables and the
- * staging/scratch buffers it grows on demand.  A context is not reentrant:
- * calls on one context must not overlap in time (use one context per host
- * thread).  Calls on different contexts may run concurrently.
+ * staging/scratch buffers it grows on demand.
+ *
+ * Threads: every call except ostft_destroy may be made on one context from
+ * several host threads at once, as the reference runner's per-connection
+ * threads call ActivateSTFT (StandAlone/cross_gpgpu/GLOBAL/RunnerCommon.cpp:
+ * 77-120).  Host-buffer calls (ostft_activate*) each take one of up to four
+ * staging lanes (own stream and device buffers; a fifth concurrent call waits
+ * for a lane); device calls queue on the caller's stream; the two-pass scratch
+ * and the batch descriptor blocks are shared, and calls queue their launches
+ * on them one at a time, ordered on the GPU by events.  ostft_last_error
+ * returns the calling thread's last error on the context.  ostft_destroy must
+ * not overlap any other call on the context.
  */
 #ifndef OCCASTFT_HIP_H
 #define OCCASTFT_HIP_H
@@ -174,7 +183,8 @@ typedef struct ostft_job {
  * and output layout: one launch per window size present (windows 2^6..2^15)
  * over a device-side queue of tiles of frames, either one workgroup per tile
  * or a persistent grid whose workgroups pull tiles from a counter; larger
- * windows run after them, request by request.  Every
+ * windows and requests of the matrix-core form (OSTFT_FLAG_MFMA_*) run after
+ * them, request by request.  Every
  * request's result is bitwise identical to running it alone with
  * ostft_run_device.  All requests are validated before anything is launched;
  * a bad one fails the call (its index is in ostft_last_error).  Queued on

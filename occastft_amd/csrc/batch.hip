@@ -10,8 +10,9 @@
 // block, so for a given set of classes they are the same every time: they are
 // captured once into a HIP graph and replayed.
 //
-// Requests of windows >= 2^15 (multi-pass path) run after the persistent
-// launches, one request at a time, on the same stream.
+// Requests of windows >= 2^16 (multi-pass path) and requests of the matrix-core
+// form run after the tile-queue launches, one request at a time, on the same
+// stream.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -64,16 +65,19 @@ struct BatchState {
     int node_persistent[2][NCLS] = {};
     hipStream_t capture = nullptr;
 
+    void drop_graph(int i)
+    {
+        if (exec[i])
+            (void)hipGraphExecDestroy(exec[i]);
+        if (graph[i])
+            (void)hipGraphDestroy(graph[i]);
+        exec[i] = nullptr;
+        graph[i] = nullptr;
+    }
     void drop_graphs()
     {
-        for (int i = 0; i < 2; ++i) {
-            if (exec[i])
-                (void)hipGraphExecDestroy(exec[i]);
-            if (graph[i])
-                (void)hipGraphDestroy(graph[i]);
-            exec[i] = nullptr;
-            graph[i] = nullptr;
-        }
+        for (int i = 0; i < 2; ++i)
+            drop_graph(i);
     }
 };
 
@@ -239,9 +243,10 @@ hipError_t ensure_slot(BatchState* st, int si, size_t bytes)
     Slot& sl = st->slot[si];
     if (sl.cap >= bytes)
         return hipSuccess;
-    // graphs name the old block: drop them (nothing in flight reads it: the
-    // caller waited on the slot's event)
-    st->drop_graphs();
+    // this slot's captured launches name the old block: drop them (nothing
+    // in flight reads it: the caller waited on the slot's event); the other
+    // slot's graph names its own block and may still be running
+    st->drop_graph(si);
     if (sl.d)
         (void)hipFree(sl.d);
     if (sl.h)
@@ -279,16 +284,13 @@ int run_batch(ostft_ctx* ctx, const ostft_job* jobs, uint32_t njobs, hipStream_t
     for (uint32_t i = 0; i < njobs; ++i) {
         const ostft_job& j = jobs[i];
         rc = ctx_prepare(ctx, &j.params, j.in, j.full_size, j.out, j.out_cap, &args[i], &lm[i]);
-        if (rc == OSTFT_OK && (j.params.flags & (OSTFT_FLAG_MFMA_FP16 | OSTFT_FLAG_MFMA_BF16)))
-            rc = ctx_fail(ctx, OSTFT_ERR_UNSUPPORTED,
-                          "the MFMA form runs through ostft_run_device and ostft_run_strided_device");
         if (rc != OSTFT_OK)
             return ctx_fail(ctx, rc, "job " + std::to_string(i) + ": " + ostft_last_error(ctx));
     }
     // 2. group by class; tiles per request
     std::vector<uint32_t> members[NCLS];
     uint64_t ntiles[NCLS] = {};
-    std::vector<uint32_t> large;
+    std::vector<uint32_t> large, mfma;
     uint32_t mask = 0;
     for (uint32_t i = 0; i < njobs; ++i) {
         if (args[i].frames == 0)
@@ -297,11 +299,18 @@ int run_batch(ostft_ctx* ctx, const ostft_job* jobs, uint32_t njobs, hipStream_t
             large.push_back(i);
             continue;
         }
+        if (jobs[i].params.flags & (OSTFT_FLAG_MFMA_FP16 | OSTFT_FLAG_MFMA_BF16)) {
+            // the matrix-core form has no tile-queue kernel: such a request
+            // runs on its own launch after the queues, with the one-request
+            // kernel (the same result as ostft_run_device)
+            mfma.push_back(i);
+            continue;
+        }
         const int c = lm[i] - LOG_M_MIN;
         rc = class_info(ctx, st, c);
         if (rc != OSTFT_OK)
             return rc;
-        const uint64_t G = (uint64_t)st->info[c].frames_per_tile;
+        const uint64_t G = (uint64_t)args[i].tile_frames;  // <= the class's frames per workgroup
         members[c].push_back(i);
         ntiles[c] += (args[i].frames + G - 1) / G;
         mask |= 1u << c;
@@ -344,13 +353,12 @@ int run_batch(ostft_ctx* ctx, const ostft_job* jobs, uint32_t njobs, hipStream_t
             hdr[c].counter = 0;
             BatchJob* bj = reinterpret_cast<BatchJob*>(sl.h + jobs_off[c]);
             uint32_t* tj = reinterpret_cast<uint32_t*>(sl.h + tiles_off[c]);
-            const uint64_t G = (uint64_t)st->info[c].frames_per_tile;
             uint64_t t = 0;
             for (size_t k = 0; k < members[c].size(); ++k) {
                 const FusedArgs& a = args[members[c][k]];
                 bj[k].a = a;
                 bj[k].tile_begin = t;
-                const uint64_t nt = (a.frames + G - 1) / G;
+                const uint64_t nt = (a.frames + a.tile_frames - 1) / a.tile_frames;
                 std::fill(tj + t, tj + t + nt, (uint32_t)k);
                 t += nt;
             }
@@ -372,6 +380,12 @@ int run_batch(ostft_ctx* ctx, const ostft_job* jobs, uint32_t njobs, hipStream_t
         if (e != hipSuccess)
             return ctx_hip_fail(ctx, e, "batch launch");
         sl.used = true;
+    }
+    for (uint32_t i : mfma) {
+        const int xf = (jobs[i].params.flags & OSTFT_FLAG_MFMA_FP16) ? XF_MFMA_FP16 : XF_MFMA_BF16;
+        const hipError_t e = launch_fused(lm[i], xf, args[i], 1, s);
+        if (e != hipSuccess)
+            return ctx_hip_fail(ctx, e, "batch MFMA launch");
     }
     for (uint32_t i : large) {
         rc = run_multipass(ctx, lm[i], args[i], s);
@@ -465,15 +479,15 @@ int autotune_batch(ostft_ctx* ctx, int exp, uint64_t samples, int* best_blocks_p
     if (e == hipSuccess)
         e = hipMalloc(&dout, sizeof(float) * out_off);
     if (e == hipSuccess)
-        e = hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(din), 0x3e800000u /* 0.25f */, in_off, nullptr);
-    if (e == hipSuccess)
         e = hipEventCreate(&e0);
     if (e == hipSuccess)
         e = hipEventCreate(&e1);
     if (e == hipSuccess)
         e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
     if (e == hipSuccess)
-        e = hipDeviceSynchronize();
+        e = hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(din), 0x3e800000u /* 0.25f */, in_off, s);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(s);
     if (e == hipSuccess) {
         for (auto& j : jobs) {
             j.in = din + reinterpret_cast<uint64_t>(j.in);

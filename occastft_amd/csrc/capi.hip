@@ -12,29 +12,56 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <condition_variable>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <tuple>
+#include <unordered_map>
 #include <vector>
 
 #include "engine.hpp"
 #include "batch.hpp"
 #include "multipass.hpp"
 
+namespace {
+// Staging of one host-buffer call (ostft_activate*): its own stream and
+// device buffers, grown on demand and kept.
+struct Lane {
+    hipStream_t stream = nullptr;
+    float* d_in = nullptr;
+    uint64_t in_cap = 0;
+    float* d_out = nullptr;
+    uint64_t out_cap = 0;
+    bool busy = false;
+};
+constexpr size_t MAX_LANES = 4;
+}  // namespace
+
+// Concurrency: any number of host threads may call into one context at once
+// (the reference runner serves each WebSocket connection on its own thread,
+// StandAlone/cross_gpgpu/GLOBAL/RunnerCommon.cpp:77-120).  Table caches are
+// built under `mu`; host-buffer calls each take a lane; the two-pass scratch
+// and the batch descriptor state are shared, and the launches that use them
+// are queued under `scratch_mu` / `batch_mu` (the GPU work itself is ordered
+// by events, not by these locks).
 struct ostft_ctx {
     int device = 0;
-    hipStream_t stream = nullptr;  // internal: table builds and host-buffer calls
-    std::mutex mu;                 // guards the table caches
+    hipStream_t stream = nullptr;  // internal: table builds
+    std::mutex mu;                 // guards the table caches and `stream`
     std::map<std::pair<int64_t, int64_t>, float2*> twiddles;  // (m, count) -> W_m^j, j < count
     std::map<std::tuple<int, int64_t, uint32_t, int>, float*> windows;  // (kind, N, sigma bits, exact)
-    float* d_in = nullptr;
-    uint64_t d_in_cap = 0;
-    float* d_out = nullptr;
-    uint64_t d_out_cap = 0;
+    std::mutex lanes_mu;
+    std::condition_variable lanes_cv;
+    std::vector<std::unique_ptr<Lane>> lanes;
+    std::mutex scratch_mu;
     ostft::Scratch scratch;
+    std::mutex batch_mu;
     ostft::BatchState* batch = nullptr;  // batch descriptors, graphs, plans (batch.hip)
-    std::string last_error;
+    std::mutex err_mu;
+    std::unordered_map<std::thread::id, std::string> errors;  // last error per calling thread
 };
 
 namespace {
@@ -56,19 +83,18 @@ struct DeviceGuard {
     }
 };
 
-int hip_fail(ostft_ctx* ctx, hipError_t e, const char* where)
-{
-    if (ctx) {
-        ctx->last_error = std::string(where) + ": " + hipGetErrorString(e);
-    }
-    return OSTFT_ERR_HIP;
-}
-
 int fail(ostft_ctx* ctx, int code, const std::string& msg)
 {
-    if (ctx)
-        ctx->last_error = msg;
+    if (ctx) {
+        std::lock_guard<std::mutex> lk(ctx->err_mu);
+        ctx->errors[std::this_thread::get_id()] = msg;
+    }
     return code;
+}
+
+int hip_fail(ostft_ctx* ctx, hipError_t e, const char* where)
+{
+    return fail(ctx, OSTFT_ERR_HIP, std::string(where) + ": " + hipGetErrorString(e));
 }
 
 bool contains(const char* s, const char* key) { return std::strstr(s, key) != nullptr; }
@@ -274,6 +300,7 @@ namespace ostft {
 // used by multipass.hip and batch.hip
 float2* ctx_twiddles(ostft_ctx* ctx, int64_t m, int64_t count, int* rc) { return get_twiddles(ctx, m, count, rc); }
 Scratch& ctx_scratch(ostft_ctx* ctx) { return ctx->scratch; }
+std::mutex& ctx_scratch_mutex(ostft_ctx* ctx) { return ctx->scratch_mu; }
 BatchState*& ctx_batch(ostft_ctx* ctx) { return ctx->batch; }
 int ctx_device(const ostft_ctx* ctx) { return ctx->device; }
 int ctx_fail(ostft_ctx* ctx, int code, const std::string& msg) { return fail(ctx, code, msg); }
@@ -319,6 +346,17 @@ int ctx_prepare(ostft_ctx* ctx, const ostft_params* params, const float* d_in, u
     a->dc = (params->flags & OSTFT_FLAG_REMOVE_DC) ? 1 : 0;
     a->in_stride = 0;
     a->out_stride = 0;
+    a->tile_frames = 1;
+    if (fused_supported(plan.log_m)) {
+        // a tile of G frames stages G N samples; a hop beyond the window (only
+        // reachable with an explicit geometry) takes fewer frames per tile, so
+        // that (frames - 1) hop + N <= G N
+        const uint64_t G = (uint64_t)fused_group_frames(plan.log_m);
+        uint64_t tf = G;
+        if (plan.g.hop > N)
+            tf = std::min<uint64_t>(G, (G - 1) * N / plan.g.hop + 1);
+        a->tile_frames = (int32_t)tf;
+    }
     a->tw = nullptr;
     a->tw_n = get_twiddles(ctx, 2 * M, M, &rc);  // W_N^k, k < M
     if (!a->tw_n)
@@ -354,7 +392,17 @@ const char* ostft_strerror(int status)
     }
 }
 
-const char* ostft_last_error(const ostft_ctx* ctx) { return ctx ? ctx->last_error.c_str() : ""; }
+const char* ostft_last_error(const ostft_ctx* cctx)
+{
+    if (!cctx)
+        return "";
+    ostft_ctx* ctx = const_cast<ostft_ctx*>(cctx);
+    std::lock_guard<std::mutex> lk(ctx->err_mu);
+    auto it = ctx->errors.find(std::this_thread::get_id());
+    // the string lives in a node of the map: its text stays put until this
+    // thread records its next error on the context
+    return it == ctx->errors.end() ? "" : it->second.c_str();
+}
 
 int ostft_window_coefficients(int32_t kind, int32_t n, float sigma, float* out)
 {
@@ -481,10 +529,14 @@ void ostft_destroy(ostft_ctx* ctx)
             (void)hipFree(kv.second);
         for (auto& kv : ctx->windows)
             (void)hipFree(kv.second);
-        if (ctx->d_in)
-            (void)hipFree(ctx->d_in);
-        if (ctx->d_out)
-            (void)hipFree(ctx->d_out);
+        for (auto& ln : ctx->lanes) {
+            if (ln->d_in)
+                (void)hipFree(ln->d_in);
+            if (ln->d_out)
+                (void)hipFree(ln->d_out);
+            if (ln->stream)
+                (void)hipStreamDestroy(ln->stream);
+        }
         ctx->scratch.destroy();
         ostft::batch_release(ctx);
         if (ctx->stream)
@@ -583,6 +635,7 @@ int ostft_run_batch_device(ostft_ctx* ctx, const ostft_job* jobs, uint32_t njobs
     DeviceGuard dg(ctx->device);
     if (!dg.ok)
         return fail(ctx, OSTFT_ERR_NO_DEVICE, "hipSetDevice failed");
+    std::lock_guard<std::mutex> lk(ctx->batch_mu);
     return ostft::run_batch(ctx, jobs, njobs, static_cast<hipStream_t>(hip_stream));
 }
 
@@ -593,6 +646,7 @@ int ostft_batch_config(ostft_ctx* ctx, int32_t exp, int32_t blocks_per_cu, int32
     DeviceGuard dg(ctx->device);
     if (!dg.ok)
         return fail(ctx, OSTFT_ERR_NO_DEVICE, "hipSetDevice failed");
+    std::lock_guard<std::mutex> lk(ctx->batch_mu);
     return ostft::batch_config(ctx, exp, blocks_per_cu, use_graphs, current);
 }
 
@@ -604,6 +658,7 @@ int ostft_autotune_batch(ostft_ctx* ctx, int32_t exp, uint64_t samples, int32_t*
     DeviceGuard dg(ctx->device);
     if (!dg.ok)
         return fail(ctx, OSTFT_ERR_NO_DEVICE, "hipSetDevice failed");
+    std::lock_guard<std::mutex> lk(ctx->batch_mu);
     return ostft::autotune_batch(ctx, exp, samples, best_blocks_per_cu, best_ms);
 }
 
@@ -615,12 +670,15 @@ int ostft_scratch_budget(ostft_ctx* ctx, uint64_t bytes, uint64_t* current)
         DeviceGuard dg(ctx->device);
         if (!dg.ok)
             return fail(ctx, OSTFT_ERR_NO_DEVICE, "hipSetDevice failed");
+        std::lock_guard<std::mutex> lk(ctx->scratch_mu);
         ctx->scratch.budget = bytes;
         if (ctx->scratch.cap * sizeof(float2) > bytes)
             ctx->scratch.release();
     }
-    if (current)
+    if (current) {
+        std::lock_guard<std::mutex> lk(ctx->scratch_mu);
         *current = ctx->scratch.budget;
+    }
     return OSTFT_OK;
 }
 
@@ -654,6 +712,54 @@ int ostft_activate_geometry(ostft_ctx* ctx, const ostft_params* params, const fl
 }  // extern "C"
 
 namespace {
+// Takes a free lane (creating one, up to MAX_LANES, or waiting for one).
+Lane* take_lane(ostft_ctx* ctx, hipError_t* err)
+{
+    std::unique_lock<std::mutex> lk(ctx->lanes_mu);
+    for (;;) {
+        for (auto& ln : ctx->lanes)
+            if (!ln->busy) {
+                ln->busy = true;
+                return ln.get();
+            }
+        if (ctx->lanes.size() < MAX_LANES) {
+            std::unique_ptr<Lane> ln(new Lane());
+            *err = hipStreamCreateWithFlags(&ln->stream, hipStreamNonBlocking);
+            if (*err != hipSuccess)
+                return nullptr;
+            ln->busy = true;
+            ctx->lanes.push_back(std::move(ln));
+            return ctx->lanes.back().get();
+        }
+        ctx->lanes_cv.wait(lk);
+    }
+}
+
+void give_lane(ostft_ctx* ctx, Lane* ln)
+{
+    {
+        std::lock_guard<std::mutex> lk(ctx->lanes_mu);
+        ln->busy = false;
+    }
+    ctx->lanes_cv.notify_one();
+}
+
+hipError_t grow(float** buf, uint64_t* cap, uint64_t need)
+{
+    if (*cap >= need)
+        return hipSuccess;
+    if (*buf)
+        (void)hipFree(*buf);
+    *buf = nullptr;
+    *cap = 0;
+    const hipError_t e = hipMalloc(buf, sizeof(float) * need);
+    if (e == hipSuccess)
+        *cap = need;
+    else
+        *buf = nullptr;
+    return e;
+}
+
 int activate_host(ostft_ctx* ctx, const ostft_params& p, const float* in, uint64_t full_size, float* out,
                   uint64_t out_cap, uint64_t* out_len, uint64_t hop, uint64_t frames)
 {
@@ -668,38 +774,34 @@ int activate_host(ostft_ctx* ctx, const ostft_params& p, const float* in, uint64
     if (!dg.ok)
         return fail(ctx, OSTFT_ERR_NO_DEVICE, "hipSetDevice failed");
     hipError_t e = hipSuccess;
-    const uint64_t in_n = full_size > 0 ? full_size : 1;
-    if (ctx->d_in_cap < in_n) {
-        if (ctx->d_in)
-            (void)hipFree(ctx->d_in);
-        ctx->d_in = nullptr;
-        ctx->d_in_cap = 0;
-        e = hipMalloc(&ctx->d_in, sizeof(float) * in_n);
-        if (e != hipSuccess)
-            return hip_fail(ctx, e, "input staging");
-        ctx->d_in_cap = in_n;
-    }
-    if (ctx->d_out_cap < plan.g.out_floats) {
-        if (ctx->d_out)
-            (void)hipFree(ctx->d_out);
-        ctx->d_out = nullptr;
-        ctx->d_out_cap = 0;
-        e = hipMalloc(&ctx->d_out, sizeof(float) * plan.g.out_floats);
-        if (e != hipSuccess)
-            return hip_fail(ctx, e, "output staging");
-        ctx->d_out_cap = plan.g.out_floats;
-    }
+    Lane* ln = take_lane(ctx, &e);
+    if (!ln)
+        return hip_fail(ctx, e, "host staging stream");
+    // the lane's buffers and stream are this call's alone until give_lane
+    struct Give {
+        ostft_ctx* c;
+        Lane* l;
+        ~Give() { give_lane(c, l); }
+    } give{ctx, ln};
+    e = grow(&ln->d_in, &ln->in_cap, full_size > 0 ? full_size : 1);
+    if (e != hipSuccess)
+        return hip_fail(ctx, e, "input staging");
+    e = grow(&ln->d_out, &ln->out_cap, plan.g.out_floats > 0 ? plan.g.out_floats : 1);
+    if (e != hipSuccess)
+        return hip_fail(ctx, e, "output staging");
     if (full_size > 0) {
-        e = hipMemcpyAsync(ctx->d_in, in, sizeof(float) * full_size, hipMemcpyHostToDevice, ctx->stream);
+        e = hipMemcpyAsync(ln->d_in, in, sizeof(float) * full_size, hipMemcpyHostToDevice, ln->stream);
         if (e != hipSuccess)
             return hip_fail(ctx, e, "H2D copy");
     }
-    rc = run_device_geometry(ctx, &p, ctx->d_in, full_size, ctx->d_out, ctx->d_out_cap, ctx->stream, hop, frames);
-    if (rc != OSTFT_OK)
+    rc = run_device_geometry(ctx, &p, ln->d_in, full_size, ln->d_out, ln->out_cap, ln->stream, hop, frames);
+    if (rc != OSTFT_OK) {
+        (void)hipStreamSynchronize(ln->stream);  // the lane's buffers may be reused at once
         return rc;
-    e = hipMemcpyAsync(out, ctx->d_out, sizeof(float) * plan.g.out_floats, hipMemcpyDeviceToHost, ctx->stream);
+    }
+    e = hipMemcpyAsync(out, ln->d_out, sizeof(float) * plan.g.out_floats, hipMemcpyDeviceToHost, ln->stream);
     if (e == hipSuccess)
-        e = hipStreamSynchronize(ctx->stream);
+        e = hipStreamSynchronize(ln->stream);
     if (e != hipSuccess)
         return hip_fail(ctx, e, "D2H copy");
     if (out_len)

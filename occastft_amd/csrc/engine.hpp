@@ -33,6 +33,8 @@ struct FusedArgs {
     int32_t combine;      // 1: full FFT; 0: stop one radix-2 stage early (odd-exponent result)
     int32_t out_mode;     // OutMode
     int32_t dc;           // subtract the frame mean after windowing
+    int32_t tile_frames;  // frames per workgroup tile of the single kernel (<= its G;
+                          // fewer when hop > N, so a tile's span fits its staging LDS)
     uint64_t in_stride;   // strided batches: floats between signal y's input and y+1's
     uint64_t out_stride;  //                  and between their outputs (blockIdx.y = y)
 };
@@ -44,6 +46,8 @@ bool mfma_supported(int log_m);
 
 // Single-kernel path: complex size M = 2^log_m, 5 <= log_m <= 14 (N = 64 .. 32768).
 bool fused_supported(int log_m);
+// frames per workgroup (G) of the single kernel at complex size 2^log_m
+int fused_group_frames(int log_m);
 // `count` signals (grid.y), signal y at a.in + y in_stride / a.out + y out_stride
 hipError_t launch_fused(int log_m, int xf, const FusedArgs& a, uint32_t count, hipStream_t s);
 

@@ -72,12 +72,14 @@ struct FusedArgsGlobal {
     const float2 OSTFT_GLOBAL* tw_n;
     float OSTFT_GLOBAL* out;
     uint64_t out_limit;
-    int32_t combine, out_mode, dc;
+    int32_t combine, out_mode, dc, tile_frames;
     uint64_t in_stride, out_stride;
 };
 static_assert(sizeof(FusedArgsGlobal) == sizeof(FusedArgs), "FusedArgsGlobal mirrors FusedArgs");
 static_assert(offsetof(FusedArgsGlobal, out) == offsetof(FusedArgs, out), "FusedArgsGlobal mirrors FusedArgs");
 static_assert(offsetof(FusedArgsGlobal, dc) == offsetof(FusedArgs, dc), "FusedArgsGlobal mirrors FusedArgs");
+static_assert(offsetof(FusedArgsGlobal, tile_frames) == offsetof(FusedArgs, tile_frames),
+              "FusedArgsGlobal mirrors FusedArgs");
 
 __device__ __forceinline__ FusedArgs from_global(const FusedArgsGlobal& g)
 {
@@ -94,6 +96,7 @@ __device__ __forceinline__ FusedArgs from_global(const FusedArgsGlobal& g)
     a.combine = g.combine;
     a.out_mode = g.out_mode;
     a.dc = g.dc;
+    a.tile_frames = g.tile_frames;
     a.in_stride = g.in_stride;
     a.out_stride = g.out_stride;
     return a;
@@ -109,8 +112,9 @@ __device__ __forceinline__ FusedArgs from_global(const FusedArgsGlobal& g)
 // keeps 16-byte rows aligned for the vector writes.
 __device__ __forceinline__ uint32_t spad(uint32_t q) { return q + ((q >> 7) << 2); }
 
-// The work of one workgroup: frames f0 .. f0+G-1 of request `a` (fewer at the
-// end of the request).  Shared by the one-request kernel and the persistent
+// The work of one workgroup: frames f0 .. f0+a.tile_frames-1 of request `a`
+// (fewer at the end of the request; tile_frames <= G, below G only when the
+// hop exceeds the window, so that the tile's span fits the staging buffer).  Shared by the one-request kernel and the persistent
 // batch kernel, so a frame's result does not depend on which of them ran it.
 template <int LOG_M, int XF = XF_FP32>
 __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t f0, float2* lds, float2* red,
@@ -121,7 +125,7 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
     float* sf = reinterpret_cast<float*>(lds);
 
     const uint32_t grp = tid / T, t = tid % T;
-    const uint32_t nf = (uint32_t)min((uint64_t)G, a.frames - f0);  // frames in this tile
+    const uint32_t nf = (uint32_t)min((uint64_t)a.tile_frames, a.frames - f0);  // frames in this tile
     const bool active = grp < nf;
     const uint64_t L = a.L, hop = a.hop;
     const float* __restrict__ in = a.in;
@@ -609,7 +613,8 @@ __global__ void __launch_bounds__(Cfg<LOG_M>::WG) __attribute__((amdgpu_waves_pe
     FusedArgs ay = a;
     ay.in += (uint64_t)blockIdx.y * a.in_stride;
     ay.out += (uint64_t)blockIdx.y * a.out_stride;
-    fused_frames<LOG_M, XF>(ay, (uint64_t)xcd_block(blockIdx.x, gridDim.x) * C::G, lds, red, threadIdx.x);
+    fused_frames<LOG_M, XF>(ay, (uint64_t)xcd_block(blockIdx.x, gridDim.x) * (uint32_t)a.tile_frames, lds, red,
+                            threadIdx.x);
 }
 
 // Batch of requests of one window size, two forms over the same tile queue
@@ -640,7 +645,7 @@ __global__ void __launch_bounds__(Cfg<LOG_M>::WG) __attribute__((amdgpu_waves_pe
     const BatchJob* jobs = reinterpret_cast<const BatchJob*>(blk + h->jobs_off);
     const uint32_t j = reinterpret_cast<const uint32_t*>(blk + h->tiles_off)[tile];
     const FusedArgs a = from_global(*reinterpret_cast<const FusedArgsGlobal*>(&jobs[j].a));
-    fused_frames<LOG_M>(a, (uint64_t)(tile - jobs[j].tile_begin) * C::G, lds, red, threadIdx.x);
+    fused_frames<LOG_M>(a, (uint64_t)(tile - jobs[j].tile_begin) * (uint32_t)a.tile_frames, lds, red, threadIdx.x);
 }
 
 template <int LOG_M>
@@ -677,7 +682,7 @@ __global__ void __launch_bounds__(Cfg<LOG_M>::WG) __attribute__((amdgpu_waves_pe
         // kernel's budget)
         uint32_t tid = threadIdx.x;
         asm volatile("" : "+v"(tid));
-        fused_frames<LOG_M>(a, (uint64_t)(tile - jobs[j].tile_begin) * C::G, lds, red, tid);
+        fused_frames<LOG_M>(a, (uint64_t)(tile - jobs[j].tile_begin) * (uint32_t)a.tile_frames, lds, red, tid);
         // every thread read `claim` before the body's first barrier, so it
         // can be overwritten now; the barrier publishes it and frees the LDS
         if (threadIdx.x == 0)
@@ -690,7 +695,9 @@ template <int LOG_M, int XF>
 static hipError_t launch_t(const FusedArgs& a, uint32_t count, hipStream_t s)
 {
     using C = Cfg<LOG_M>;
-    const uint64_t blocks = (a.frames + C::G - 1) / C::G;
+    if (a.tile_frames < 1 || a.tile_frames > C::G)
+        return hipErrorInvalidValue;
+    const uint64_t blocks = (a.frames + a.tile_frames - 1) / a.tile_frames;
     if (blocks == 0 || count == 0)
         return hipSuccess;
     if (blocks > 0x7fffffffull || count > 65535u)
@@ -752,6 +759,23 @@ static const void* batch_fn_t(bool persistent)
 }
 
 bool fused_supported(int log_m) { return log_m >= 5 && log_m <= 14; }
+
+int fused_group_frames(int log_m)
+{
+    switch (log_m) {
+    case 5: return Cfg<5>::G;
+    case 6: return Cfg<6>::G;
+    case 7: return Cfg<7>::G;
+    case 8: return Cfg<8>::G;
+    case 9: return Cfg<9>::G;
+    case 10: return Cfg<10>::G;
+    case 11: return Cfg<11>::G;
+    case 12: return Cfg<12>::G;
+    case 13: return Cfg<13>::G;
+    case 14: return Cfg<14>::G;
+    default: return 1;
+    }
+}
 
 hipError_t launch_batch(int log_m, const BatchClassArgs& b, uint32_t grid, bool persistent, hipStream_t s)
 {

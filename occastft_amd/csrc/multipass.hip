@@ -72,7 +72,10 @@ hipError_t Scratch::handoff(hipStream_t s)
 void Scratch::release()
 {
     if (a) {
-        (void)hipDeviceSynchronize();
+        // the last call's launches (on whatever stream) must be done with it;
+        // other streams and contexts on the device are not waited for
+        if (used && done)
+            (void)hipEventSynchronize(done);
         (void)hipFree(a);
     }
     a = nullptr;
@@ -379,6 +382,9 @@ int run_multipass(ostft_ctx* ctx, int log_m, const FusedArgs& a, hipStream_t s)
     if (multipass_plan(log_m, lr) != 2)
         return OSTFT_ERR_UNSUPPORTED;
     const uint64_t M = 1ull << log_m;
+    // one call at a time queues launches on the shared scratch; the event
+    // chain (acquire / handoff) orders their GPU work across streams
+    std::lock_guard<std::mutex> lk(ctx_scratch_mutex(ctx));
     Scratch& sc = ctx_scratch(ctx);
     // the fewest chunks the budget allows, of equal size (a request of
     // 2^k + 1 frames gets no one-frame chunk at its end)

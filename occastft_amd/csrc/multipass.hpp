@@ -1,6 +1,8 @@
 // multipass.hpp -- large-window path (complex transform sizes 2^13 .. 2^19).
 #pragma once
 
+#include <mutex>
+
 #include "engine.hpp"
 
 struct ostft_ctx;
@@ -21,13 +23,14 @@ struct Scratch {
     hipError_t ensure(uint64_t n);
     hipError_t acquire(hipStream_t s);  // order stream s after the previous user
     hipError_t handoff(hipStream_t s);  // record the end of this call's launches
-    void release();                     // frees the buffer (waits for the device)
+    void release();                     // frees the buffer (after the last user's launches)
     void destroy();                     // release + the event
 };
 
 // accessors into the context (defined in capi.hip)
 float2* ctx_twiddles(ostft_ctx* ctx, int64_t m, int64_t count, int* rc);
 Scratch& ctx_scratch(ostft_ctx* ctx);
+std::mutex& ctx_scratch_mutex(ostft_ctx* ctx);  // held while a call queues its scratch launches
 
 // Runs the two passes (the split epilogue inside the last).  Returns an OSTFT_* status.
 int run_multipass(ostft_ctx* ctx, int log_m, const FusedArgs& a, hipStream_t s);

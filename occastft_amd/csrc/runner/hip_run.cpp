@@ -37,10 +37,17 @@ namespace {
 
 using ostft_wire::Request;
 
-// page-rounded byte size of n floats (FFTStruct.hpp:138-145)
+// A Cap'n Proto list holds fewer than 2^29 elements: the largest result the
+// data list can carry.
+constexpr uint64_t MAX_LIST_FLOATS = (1ull << 29) - 1;
+
+// page-rounded byte size of n floats (FFTStruct.hpp:138-145); 0 when it does
+// not fit in 64 bits
 uint64_t page_bytes(uint64_t nfloats)
 {
     const uint64_t page = (uint64_t)sysconf(_SC_PAGESIZE);
+    if (nfloats > (UINT64_MAX - page) / sizeof(float))
+        return 0;
     const uint64_t bytes = nfloats * sizeof(float);
     return ((bytes + page - 1) / page) * page;
 }
@@ -98,17 +105,13 @@ public:
     // concurrent connections share batches on the GPU; a request whose input
     // and output do not fit in the server's arena runs on its own through a
     // context of its own.
+    // `max_out`: the most floats the reply can carry (the client's shared
+    // memory, or a data list); a larger result is refused before anything is
+    // allocated for it.
     bool ActivateSTFT(const std::vector<float>& in, int exp, float overlap, const std::string& options,
-                      std::vector<float>& out, std::string& why)
+                      uint64_t max_out, std::vector<float>& out, std::string& why)
     {
-        {
-            std::unique_lock<std::mutex> lk(ready_mu_);
-            ready_cv_.wait(lk, [this] { return ready_; });
-        }
-        if (!srv_) {
-            why = "no HIP device";
-            return false;
-        }
+        // the request itself is checked first: it needs no device
         ostft_params p;
         int rc = ostft_parse_options(options.c_str(), exp, overlap, &p);
         ostft_geometry g;
@@ -116,6 +119,19 @@ public:
             rc = ostft_get_geometry(&p, in.size(), &g);
         if (rc != OSTFT_OK) {
             why = ostft_strerror(rc);
+            return false;
+        }
+        if (g.out_floats > max_out) {
+            why = "result of " + std::to_string(g.out_floats) + " floats exceeds what the reply carries ("
+                  + std::to_string(max_out) + ")";
+            return false;
+        }
+        {
+            std::unique_lock<std::mutex> lk(ready_mu_);
+            ready_cv_.wait(lk, [this] { return ready_; });
+        }
+        if (!srv_) {
+            why = "no HIP device";
             return false;
         }
         out.resize(g.out_floats ? g.out_floats : 1);
@@ -186,7 +202,7 @@ private:
                 return false;
             }
             shm.bytes = page_bytes(req.overlapdataLength);
-            if (shm.bytes == 0 || req.dataLength * sizeof(float) > shm.bytes) {
+            if (shm.bytes == 0 || req.dataLength > shm.bytes / sizeof(float)) {
                 why = "shared memory smaller than the input";
                 return false;
             }
@@ -217,7 +233,27 @@ private:
             }
             return;
         }
-        Request req, rep;
+        Request rep;
+        std::string out;
+        try {
+            out = Handle(msg, rep);
+        } catch (const std::exception& e) {
+            // an allocation or a library call that throws ends this request
+            // with an ERR reply, not the runner (other clients keep their
+            // connections)
+            std::cerr << "hipRun: request " << rep.mappedID << " failed: " << e.what() << std::endl;
+            rep.sharedMemory = "ERR";
+            out.clear();
+            ostft_wire::encode_request(rep, nullptr, 0, out);
+        }
+        conn.send_binary(out);
+    }
+
+    // Decodes one binary message and runs it; returns the encoded reply.
+    // `rep` receives the reply fields as soon as they are known.
+    std::string Handle(std::string& msg, Request& rep)
+    {
+        Request req;
         std::string err;
         const bool decoded
             = ostft_wire::decode_request(reinterpret_cast<const uint8_t*>(msg.data()), msg.size(), req, err);
@@ -241,10 +277,11 @@ private:
             ok = AccessData(req, data, shm, err);
             req.data.clear();
             req.data.shrink_to_fit();
+            const uint64_t max_out = shm.ptr ? shm.bytes / sizeof(float) : MAX_LIST_FLOATS;
             if (ok)
-                ok = ActivateSTFT(data, (int)req.windowRadix, req.overlapRatio, req.options, result, err);
+                ok = ActivateSTFT(data, (int)req.windowRadix, req.overlapRatio, req.options, max_out, result, err);
             if (ok && shm.ptr) {
-                if (result.size() * sizeof(float) > shm.bytes) {
+                if (result.size() > shm.bytes / sizeof(float)) {
                     ok = false;
                     err = "result larger than the shared memory";
                 } else {
@@ -263,9 +300,10 @@ private:
         std::string out;
         if (!ostft_wire::encode_request(rep, result.data(), result.size(), out)) {
             rep.sharedMemory = "ERR";
+            out.clear();
             ostft_wire::encode_request(rep, nullptr, 0, out);
         }
-        conn.send_binary(out);
+        return out;
     }
 
     ostft_wire::WsServer server_;

@@ -11,6 +11,8 @@
 #include <cctype>
 #include <cerrno>
 #include <cstring>
+#include <new>
+#include <stdexcept>
 
 namespace ostft_wire {
 namespace {
@@ -318,12 +320,23 @@ void WsServer::connection(int fd, Handler h)
                 if (!read_exact(fd, mask, 4))
                     break;
                 const bool control = op >= 8;
-                if ((control && (len > 125 || !fin)) || msg.size() + len > max_message_) {
+                // the message so far is at most max_message_ long, so the
+                // room left cannot wrap (msg.size() + len could, for a 64-bit len)
+                if ((control && (len > 125 || !fin)) || len > max_message_ - msg.size()) {
                     const unsigned char code[2] = {0x03, (unsigned char)(control ? 0xEA : 0xF1)};  // 1002 / 1009
                     conn->send(8, code, 2);
                     break;
                 }
-                std::string payload((size_t)len, '\0');
+                std::string payload;
+                try {
+                    payload.assign((size_t)len, '\0');
+                    if (op == 0)
+                        msg.reserve(msg.size() + (size_t)len);
+                } catch (const std::bad_alloc&) {
+                    const unsigned char code[2] = {0x03, 0xF1};  // 1009: message too big
+                    conn->send(8, code, 2);
+                    break;
+                }
                 if (len && !read_exact(fd, &payload[0], (size_t)len))
                     break;
                 for (uint64_t i = 0; i < len; ++i)
@@ -361,7 +374,14 @@ void WsServer::connection(int fd, Handler h)
                 if (fin) {
                     const int done_op = msg_op;
                     msg_op = 0;
-                    h(*conn, done_op, msg);
+                    try {
+                        h(*conn, done_op, msg);
+                    } catch (const std::exception&) {
+                        // a handler that throws closes this connection only
+                        const unsigned char code[2] = {0x03, 0xF3};  // 1011: internal error
+                        conn->send(8, code, 2);
+                        break;
+                    }
                     msg.clear();
                     if (stopping_)
                         break;

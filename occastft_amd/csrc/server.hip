@@ -217,7 +217,7 @@ struct ostft_server {
         r->t_done = now;
         if (status == OSTFT_OK) {
             ++st.completed;
-            st.frames += r->g.frames;
+            st.frames += r->g.compute_frames;
             st.bytes_in += sizeof(float) * r->L;
             st.bytes_out += sizeof(float) * r->g.out_floats;
             lat_ms.push_back(std::chrono::duration<double, std::milli>(now - r->t_submit).count());
@@ -349,7 +349,10 @@ void ostft_server::scheduler_loop()
             }
         }
         if (rc != OSTFT_OK) {
-            (void)hipDeviceSynchronize();  // nothing of the batch may still run
+            // nothing of the batch may still run when its chunk is freed: wait
+            // for this server's streams (not for other work on the device)
+            for (hipStream_t st : {h2d, compute, d2h})
+                (void)hipStreamSynchronize(st);
             raw->status = rc;
         }
         lk.lock();
@@ -484,7 +487,8 @@ void ostft_server_destroy(ostft_server* srv)
     srv->sched.join();
     srv->comp.join();
     (void)hipSetDevice(srv->device);
-    (void)hipDeviceSynchronize();
+    for (hipStream_t st : {srv->h2d, srv->compute, srv->d2h})
+        (void)hipStreamSynchronize(st);
     for (hipEvent_t e : srv->event_pool)
         (void)hipEventDestroy(e);
     (void)hipFree(srv->arena_d);

@@ -152,6 +152,65 @@ def test_fragmented_request(runner):
         assert rep.sharedMemory == "ERR"
 
 
+def _still_serving(runner):
+    """A fresh connection completes CLOSE_REQUEST and the process exits 0:
+    the runner survived whatever came before."""
+    c = runner.client()
+    c.send_text("CLOSE_REQUEST")
+    assert c.recv_message() == (1, b"CLOSE_COMPLETE")
+    runner.proc.wait(timeout=30)
+    assert runner.proc.returncode == 0
+    return runner.proc.stderr.read().decode()
+
+
+def test_oversized_result_is_refused_before_allocation(runner):
+    """overlap 0.9999999 at window 2^20 over 2^20 samples gives ~8.4e6 frames,
+    ~8.8e12 output floats: more than a reply's data list carries (< 2^29).
+    The request must come back as ERR (not abort the runner on std::bad_alloc)."""
+    x = np.zeros(1 << 20, np.float32)
+    c = runner.client()
+    c.send_binary(wc.encode(wc.Req(mappedID="big", windowRadix=20, overlapRatio=0.9999999, dataLength=len(x),
+                                   data=x)))
+    op, p = c.recv_message()
+    rep = wc.decode(p)
+    assert op == 2 and rep.sharedMemory == "ERR" and rep.mappedID == "big"
+    err = _still_serving(runner)
+    assert "exceeds what the reply carries" in err
+
+
+def test_continuation_length_near_2_64_closes_only_that_connection(runner):
+    """A continuation frame whose 64-bit length would wrap msg.size() + len is
+    refused with close code 1009; the runner keeps serving."""
+    c = runner.client()
+    c.send_frame(2, b"\x00" * 16, fin=False)
+    key = b"\x01\x02\x03\x04"
+    c.s.sendall(bytes([0x80, 0x80 | 127]) + ((1 << 64) - 8).to_bytes(8, "big") + key)
+    op, p = c.recv_message()
+    assert op == 8 and p[:2] == (1009).to_bytes(2, "big")
+    _still_serving(runner)
+
+
+@pytest.mark.parametrize("data_len,olen", [(1 << 62, 1024), (16, 1 << 63), (1 << 61, (1 << 62) + 3)])
+def test_shared_memory_sizes_that_overflow_are_refused(runner, data_len, olen):
+    """dataLength * 4 or the page rounding of overlapdataLength * 4 would wrap
+    in 64 bits: an ERR reply, not a resize that throws."""
+    from multiprocessing import shared_memory
+
+    name = f"STFTovf{os.getpid()}SHAREDMEM"
+    shm = shared_memory.SharedMemory(name=name, create=True, size=4096)
+    try:
+        c = runner.client()
+        c.send_binary(wc.encode(wc.Req(sharedMemory="/" + name, mappedID="ovf", windowRadix=10, overlapRatio=0.5,
+                                       dataLength=data_len, overlapdataLength=olen)))
+        op, p = c.recv_message()
+        rep = wc.decode(p)
+        assert op == 2 and rep.sharedMemory == "ERR" and rep.mappedID == "ovf"
+        _still_serving(runner)
+    finally:
+        shm.close()
+        shm.unlink()
+
+
 def test_runner_death_is_visible_to_the_client(runner):
     """The client's fallback trigger (STFTProxy.cpp:59-72): the connection
     drops when the runner process dies."""

@@ -6,7 +6,8 @@ pipeline (oracle/fp64.c), the same bar as the FP32 path, for every window kind
 and output layout on windows 64, 256, 1024, including signals scaled by 1e-20
 and 1e20 (every GEMM column is scaled by a power of two from its own maximum)
 and all-zero signals.  The BF16 form carries 16 significant bits per operand
-pair: its error against FP64 is measured and bounded at 1e-4, not 1e-5.
+pair (about 1.5e-5 per term at worst, 5-7e-6 measured per frame): it is held
+to the same 1e-5 bar on noise signals.
 """
 import numpy as np
 import pytest
@@ -69,8 +70,6 @@ def check_vs_fp64(runner, x, p, tol):
     r = np.asarray(r, np.float64)[: y.size]
     assert y.size == g["out_floats"]
     mx, p50, zabs = frame_errors(y, r, per_frame(p))
-    if p.output == N.OUT_POWER:
-        tol = 2 * tol  # |X|^2 doubles the relative error of |X|
     assert mx <= tol and zabs <= 1e-6 * max(1.0, float(np.abs(x).max())), \
         (p.exp, p.window, p.output, float(np.abs(x).max()), mx)
     return mx
@@ -113,7 +112,7 @@ def test_bf16_form_error_is_bounded(runner, exp):
     x = np.random.default_rng(exp).uniform(-1, 1, 40000).astype(np.float32)
     p = ostft.make_params(exp, 0.5, N.WIN_HANNING, 0.0, N.OUT_MAGNITUDE,
                           N.FLAG_REMOVE_DC | N.FLAG_MFMA_BF16 | EXACT)
-    mx = check_vs_fp64(runner, x, p, 1e-4)
+    mx = check_vs_fp64(runner, x, p, 1e-5)
     assert mx > 1e-7  # it is the 16-bit form, not the FP32 path
 
 
@@ -154,7 +153,7 @@ def test_config4_full_size(runner):
     """4096 signals x 2^18 samples of unit-variance 1/f noise, windows 64 / 256
     / 1024, 50%, Hann, half spectrum, one strided launch per form: every frame
     of four signals against the FP64 pipeline (FP32 and FP16 forms <= 1e-5,
-    BF16 <= 1e-4)."""
+    BF16 <= 1e-5)."""
     import torch
 
     S, L = 4096, 1 << 18
@@ -170,7 +169,7 @@ def test_config4_full_size(runner):
     rows = (0, 1, 2047, 4095)
     xs = {r: x[r].cpu().numpy() for r in rows}
     for exp in EXPS:
-        for flags, tol in ((0, 1e-5), (N.FLAG_MFMA_FP16, 1e-5), (N.FLAG_MFMA_BF16, 1e-4)):
+        for flags, tol in ((0, 1e-5), (N.FLAG_MFMA_FP16, 1e-5), (N.FLAG_MFMA_BF16, 1e-5)):
             p = ostft.make_params(exp, 0.5, N.WIN_HANNING, 0.0, N.OUT_HALF_SPECTRUM, EXACT | flags)
             out = runner.run_strided_device(x, p)
             torch.cuda.synchronize()

@@ -71,7 +71,7 @@ def test_golden_matches_fp64_closed_forms(golden):
 # ---------------------------------------------------------------- live reference
 
 @pytest.mark.skipif(not po.have_ref(), reason="oracle/_ref not built (no reference checkout here)")
-@pytest.mark.parametrize("exp", [6, 7, 8, 9, 10, 11, 12, 13, 14])
+@pytest.mark.parametrize("exp", [6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16])
 def test_restatement_matches_reference_kernels_live(exp, rng):
     for ov in (0.0, 0.5, 0.6):
         for opts in ("", "--hanning_window --remove_dc", "--gaussian_window=0.45<<sigma --half_complex_return",

@@ -133,7 +133,7 @@ def test_hiprun_killed_falls_back_to_cpu():
         assert rc == 0
         y = p.request(x, 10, 0.5, OPTS)
         assert y is not None and y.size == r.size
-        assert frame_errors(y, r, 512)[0] <= 1e-5
+        assert frame_errors(y, r, 1 << 10)[0] <= 1e-5  # per 1024-sample frame
 
         # fault injection: SIGKILL the runner; the next request finds its
         # connection gone, is lost, and the proxy moves to the CPU runner

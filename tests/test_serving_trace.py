@@ -1,0 +1,112 @@
+"""The ragged serving workload at its stated size, through the continuous-
+batching server (ostft_server_*).
+
+2000 requests arriving as a Poisson process at 2000 requests/s, each a float32
+signal of length log-uniform in [2^14, 2^22] (U[-1,1] noise plus one random
+sinusoid), window uniform over 2^8..2^13, overlap over {0, 25, 50, 75}%,
+window type over Hanning / Hamming / Blackman, power spectrum (N/2+1 bins).
+Requests are submitted on their arrival schedule from one thread while the
+server's earlier batches run, so arrivals coalesce into batches as they come.
+
+Checks: every result is bitwise the result of the same request run alone
+(ostft_run_device), and 24 sampled requests are within 1e-5 per frame of the
+FP64 pipeline (oracle/fp64.c).  This replaces the reference runner's
+per-request path (StandAlone/cross_gpgpu/GLOBAL/RunnerCommon.cpp:77-120).
+"""
+import queue
+import threading
+import time
+
+import numpy as np
+import pytest
+
+from conftest import frame_errors
+
+import occastft_amd as ostft
+
+import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+REQUESTS = 2000
+RATE = 2000.0  # arrivals per second
+EXPS = tuple(range(8, 14))
+OVERLAPS = (0.0, 0.25, 0.5, 0.75)
+WINDOWS = (("--hanning_window", "hanning"), ("--hamming_window", "hamming"), ("--blackman_window", "blackman"))
+
+
+def make_trace(seed):
+    import torch
+
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0 / RATE, REQUESTS)
+    arrivals = np.cumsum(gaps) - gaps[0]
+    gen = torch.Generator(device="cuda").manual_seed(seed)
+    reqs = []
+    for i in range(REQUESTS):
+        L = int(np.exp(rng.uniform(np.log(2.0 ** 14), np.log(2.0 ** 22))))
+        exp = int(rng.choice(EXPS))
+        ov = float(rng.choice(OVERLAPS))
+        wopt, wname = WINDOWS[int(rng.integers(0, len(WINDOWS)))]
+        f, a, ph = rng.uniform(0, 0.5), rng.uniform(0.1, 1.0), rng.uniform(0, 2 * np.pi)
+        t = torch.arange(L, dtype=torch.float64, device="cuda")
+        sig = (torch.rand(L, dtype=torch.float64, device="cuda", generator=gen) * 2 - 1
+               + a * torch.sin(2 * np.pi * f * t + ph)).float()
+        reqs.append(dict(t=float(arrivals[i]), x=sig.cpu().numpy(), exp=exp, ov=ov, wname=wname,
+                         p=ostft.parse_options(wopt + " --power_return", exp, ov)))
+    return reqs
+
+
+def test_ragged_poisson_trace_2000_requests():
+    import torch
+
+    assert torch.cuda.is_available(), "GPU tests need a GPU"
+    reqs = make_trace(0x5EED0005)
+    outs = [np.empty(ostft.geometry(r["p"], r["x"].size)["out_floats"], np.float32) for r in reqs]
+    print(f"trace: {REQUESTS} requests, {sum(r['x'].size for r in reqs) / 1e9:.2f} G samples", flush=True)
+
+    with ostft.Server(0, arena_bytes=2 << 30) as srv:
+        q = queue.Queue()
+
+        def submitter():
+            t0 = time.perf_counter()
+            for r, o in zip(reqs, outs):
+                dt = t0 + r["t"] - time.perf_counter()
+                if dt > 0:
+                    time.sleep(dt)
+                q.put(srv.submit(r["x"], r["p"], out=o))
+            q.put(None)
+
+        th = threading.Thread(target=submitter)
+        th.start()
+        got = []
+        while True:
+            tk = q.get()
+            if tk is None:
+                break
+            got.append(srv.wait(tk))
+        th.join()
+        st = srv.stats()
+    print(f"served: {st['completed']} ok, {st['failed']} failed, {st['batches']} batches "
+          f"(mean {st['mean_batch']:.2f} per batch), p50 {st['latency_p50_ms']:.3f} ms, "
+          f"p99 {st['latency_p99_ms']:.3f} ms", flush=True)
+    assert st["completed"] == REQUESTS and st["failed"] == 0
+    assert st["batches"] < REQUESTS  # arrivals coalesced into shared batches
+
+    # every request: bitwise its run alone
+    with ostft.Runner(0) as runner:
+        for i, (r, g) in enumerate(zip(reqs, got)):
+            y = runner.run_device(torch.from_numpy(r["x"]).cuda(), r["p"])
+            torch.cuda.synchronize()
+            a = y.cpu().numpy()
+            assert g.shape == a.shape and np.array_equal(g.view(np.uint32), a.view(np.uint32)), \
+                f"request {i}: exp {r['exp']} ov {r['ov']} L {r['x'].size}"
+
+    # sampled requests against the FP64 pipeline
+    rng = np.random.default_rng(1)
+    for i in sorted(rng.choice(REQUESTS, 24, replace=False).tolist()):
+        r = reqs[i]
+        g = ostft.geometry(r["p"], r["x"].size)
+        t = po.stft64(r["x"], r["exp"], g["hop"], g["frames"], po.WINDOWS[r["wname"]], 0.0, False, po.OUT_POWER)
+        mx, _, zabs = frame_errors(got[i], t, (1 << r["exp"]) // 2 + 1)
+        assert mx <= 1e-5 and zabs <= 1e-6, (i, r["exp"], r["ov"], r["wname"], mx, zabs)
