@@ -53,6 +53,8 @@ extern "C" {
 #define OSTFT_ERR_TOO_LARGE 7      /* sizes beyond what the engine addresses */
 #define OSTFT_ERR_TIMEOUT 8        /* ostft_server_wait: the request is still running */
 #define OSTFT_ERR_SHUTDOWN 9       /* ostft_server_submit after shutdown began */
+#define OSTFT_ERR_NOT_PREPARED 10  /* ostft_run_device_noalloc: a window or twiddle table of
+                                      the request is not resident in the context yet */
 
 /* window kinds, in Default_Pipeline's precedence order (RunnerCommon.cpp:216-247) */
 enum ostft_window {
@@ -148,6 +150,24 @@ void ostft_destroy(ostft_ctx* ctx);
  * Replaces the device part of Runner::ActivateSTFT. */
 int ostft_run_device(ostft_ctx* ctx, const ostft_params* params, const float* d_in,
                      uint64_t full_size, float* d_out, uint64_t out_cap, void* hip_stream);
+
+/* ostft_run_device that allocates nothing, for callers that cannot call
+ * hipMalloc (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid allocation).  No counterpart in the reference.  Same arguments,
+ * kernels and results (bitwise) as ostft_run_device, with two differences:
+ *   - the window and twiddle tables of the request must already be resident
+ *     in the context (one ostft_run_device or ostft_activate call with the
+ *     same window kind, size and sigma builds them); otherwise the call
+ *     returns OSTFT_ERR_NOT_PREPARED and launches nothing;
+ *   - windows 2^16..2^20 pass their intermediate spectra through a fixed
+ *     64 MiB device array per GPU (frames in chunks of 64 MiB / (4 N) bytes:
+ *     16 frames at 2^20) instead of the context's growable scratch, with no
+ *     event.  Every call on a GPU shares that array and it keeps the last
+ *     call's contents (each call overwrites what it reads first): calls that
+ *     overlap in time on one GPU race; keep them on one stream or ordered.
+ * Windows up to 2^15 use no scratch. */
+int ostft_run_device_noalloc(ostft_ctx* ctx, const ostft_params* params, const float* d_in,
+                             uint64_t full_size, float* d_out, uint64_t out_cap, void* hip_stream);
 
 /* Strided batch of `count` equal requests: signal y is d_in + y * in_stride
  * (full_size floats), its result goes to d_out + y * out_stride (out_stride a

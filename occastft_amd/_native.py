@@ -22,6 +22,7 @@ ERR_NO_DEVICE = 6
 ERR_TOO_LARGE = 7
 ERR_TIMEOUT = 8
 ERR_SHUTDOWN = 9
+ERR_NOT_PREPARED = 10
 
 # windows
 WIN_NONE, WIN_HANNING, WIN_HAMMING, WIN_BLACKMAN, WIN_NUTTALL = 0, 1, 2, 3, 4
@@ -45,6 +46,7 @@ EXPORTS = (
     "ostft_create",
     "ostft_destroy",
     "ostft_run_device",
+    "ostft_run_device_noalloc",
     "ostft_activate",
     "ostft_strerror",
     "ostft_last_error",
@@ -163,6 +165,8 @@ def lib() -> ctypes.CDLL:
     L.ostft_destroy.argtypes = [vp]
     L.ostft_run_device.restype = ctypes.c_int
     L.ostft_run_device.argtypes = [vp, ctypes.POINTER(Params), vp, ctypes.c_uint64, vp, ctypes.c_uint64, vp]
+    L.ostft_run_device_noalloc.restype = ctypes.c_int
+    L.ostft_run_device_noalloc.argtypes = [vp, ctypes.POINTER(Params), vp, ctypes.c_uint64, vp, ctypes.c_uint64, vp]
     L.ostft_activate.restype = ctypes.c_int
     L.ostft_activate.argtypes = [vp, fp, ctypes.c_uint64, ctypes.c_int32, ctypes.c_float, ctypes.c_char_p, fp,
                                  ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)]

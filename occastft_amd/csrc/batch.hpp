@@ -18,7 +18,8 @@ int ctx_device(const ostft_ctx* ctx);
 int ctx_fail(ostft_ctx* ctx, int code, const std::string& msg);
 int ctx_hip_fail(ostft_ctx* ctx, hipError_t e, const char* where);
 int ctx_prepare(ostft_ctx* ctx, const ostft_params* params, const float* d_in, uint64_t full_size, float* d_out,
-                uint64_t out_cap, FusedArgs* a, int* log_m, uint64_t hop = 0, uint64_t frames = 0);
+                uint64_t out_cap, FusedArgs* a, int* log_m, uint64_t hop = 0, uint64_t frames = 0,
+                bool noalloc = false);
 
 // The batch entry points behind the C ABI (the context's device is current).
 int run_batch(ostft_ctx* ctx, const ostft_job* jobs, uint32_t njobs, hipStream_t s);

@@ -182,14 +182,25 @@ int make_plan(const ostft_params* p, uint64_t L, Plan* plan, ostft_ctx* ctx, uin
     return OSTFT_OK;
 }
 
-// W_m^j for j < count, generated once and kept.
-float2* get_twiddles(ostft_ctx* ctx, int64_t m, int64_t count, int* rc)
+int not_prepared(ostft_ctx* ctx, const char* what)
+{
+    return fail(ctx, OSTFT_ERR_NOT_PREPARED,
+                std::string(what) + " of this window kind and size is not resident yet: run the request once "
+                                    "through ostft_run_device (or ostft_activate) first");
+}
+
+// W_m^j for j < count, generated once and kept (with `noalloc`: looked up only).
+float2* get_twiddles(ostft_ctx* ctx, int64_t m, int64_t count, int* rc, bool noalloc = false)
 {
     std::lock_guard<std::mutex> lk(ctx->mu);
     const auto key = std::make_pair(m, count);
     auto it = ctx->twiddles.find(key);
     if (it != ctx->twiddles.end())
         return it->second;
+    if (noalloc) {
+        *rc = not_prepared(ctx, "a twiddle table");
+        return nullptr;
+    }
     float2* t = nullptr;
     hipError_t e = hipMalloc(&t, sizeof(float2) * (size_t)count);
     if (e == hipSuccess)
@@ -256,8 +267,8 @@ float compat_window(int kind, int index, int windowSize, float sigma)
 }
 
 // Window table for (kind, n, sigma): the reference's coefficients, or with
-// `exact` the FP64 formulas generated on the GPU.
-float* get_window(ostft_ctx* ctx, int kind, int64_t n, float sigma, int exact, int* rc)
+// `exact` the FP64 formulas generated on the GPU (with `noalloc`: looked up only).
+float* get_window(ostft_ctx* ctx, int kind, int64_t n, float sigma, int exact, int* rc, bool noalloc = false)
 {
     uint32_t sb = 0;
     std::memcpy(&sb, &sigma, sizeof(sb));
@@ -268,6 +279,10 @@ float* get_window(ostft_ctx* ctx, int kind, int64_t n, float sigma, int exact, i
     auto it = ctx->windows.find(key);
     if (it != ctx->windows.end())
         return it->second;
+    if (noalloc) {
+        *rc = not_prepared(ctx, "the window table");
+        return nullptr;
+    }
     float* w = nullptr;
     hipError_t e = hipMalloc(&w, sizeof(float) * (size_t)n);
     if (e == hipSuccess) {
@@ -298,7 +313,10 @@ float* get_window(ostft_ctx* ctx, int kind, int64_t n, float sigma, int exact, i
 
 namespace ostft {
 // used by multipass.hip and batch.hip
-float2* ctx_twiddles(ostft_ctx* ctx, int64_t m, int64_t count, int* rc) { return get_twiddles(ctx, m, count, rc); }
+float2* ctx_twiddles(ostft_ctx* ctx, int64_t m, int64_t count, int* rc, bool noalloc)
+{
+    return get_twiddles(ctx, m, count, rc, noalloc);
+}
 Scratch& ctx_scratch(ostft_ctx* ctx) { return ctx->scratch; }
 std::mutex& ctx_scratch_mutex(ostft_ctx* ctx) { return ctx->scratch_mu; }
 BatchState*& ctx_batch(ostft_ctx* ctx) { return ctx->batch; }
@@ -309,7 +327,7 @@ int ctx_hip_fail(ostft_ctx* ctx, hipError_t e, const char* where) { return hip_f
 // Validates one device-resident request and resolves its tables (window,
 // twiddles): everything a launch needs, in `a`.  The caller's device is current.
 int ctx_prepare(ostft_ctx* ctx, const ostft_params* params, const float* d_in, uint64_t full_size, float* d_out,
-                uint64_t out_cap, FusedArgs* a, int* log_m, uint64_t hop, uint64_t frames)
+                uint64_t out_cap, FusedArgs* a, int* log_m, uint64_t hop, uint64_t frames, bool noalloc)
 {
     if (!params || !d_out || (!d_in && full_size > 0))
         return fail(ctx, OSTFT_ERR_INVALID_ARG, "null argument");
@@ -329,7 +347,7 @@ int ctx_prepare(ostft_ctx* ctx, const ostft_params* params, const float* d_in, u
     const float* w = nullptr;
     if (win_kind != OSTFT_WIN_NONE) {
         w = get_window(ctx, win_kind, (int64_t)N, params->sigma, (params->flags & OSTFT_FLAG_EXACT_WINDOW) ? 1 : 0,
-                       &rc);
+                       &rc, noalloc);
         if (!w)
             return rc;
     }
@@ -358,11 +376,11 @@ int ctx_prepare(ostft_ctx* ctx, const ostft_params* params, const float* d_in, u
         a->tile_frames = (int32_t)tf;
     }
     a->tw = nullptr;
-    a->tw_n = get_twiddles(ctx, 2 * M, M, &rc);  // W_N^k, k < M
+    a->tw_n = get_twiddles(ctx, 2 * M, M, &rc, noalloc);  // W_N^k, k < M
     if (!a->tw_n)
         return rc;
     if (fused_supported(plan.log_m)) {
-        a->tw = get_twiddles(ctx, M, M, &rc);
+        a->tw = get_twiddles(ctx, M, M, &rc, noalloc);
         if (!a->tw)
             return rc;
     }
@@ -388,6 +406,7 @@ const char* ostft_strerror(int status)
     case OSTFT_ERR_TOO_LARGE: return "request too large";
     case OSTFT_ERR_TIMEOUT: return "request still running";
     case OSTFT_ERR_SHUTDOWN: return "server is shutting down";
+    case OSTFT_ERR_NOT_PREPARED: return "tables not resident (allocation-free call)";
     default: return "unknown status";
     }
 }
@@ -510,7 +529,13 @@ int ostft_create(int32_t device, ostft_ctx** out)
     ostft_ctx* ctx = new ostft_ctx();
     ctx->device = device;
     hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    // the fixed scratch of the allocation-free entry point: its address on
+    // this device, looked up once per device with the device current
+    if (e == hipSuccess)
+        e = ostft::multipass_static_scratch_init(device);
     if (e != hipSuccess) {
+        if (ctx->stream)
+            (void)hipStreamDestroy(ctx->stream);
         delete ctx;
         return OSTFT_ERR_HIP;
     }
@@ -580,6 +605,30 @@ int ostft_run_device(ostft_ctx* ctx, const ostft_params* params, const float* d_
     if (!dg.ok)
         return fail(ctx, OSTFT_ERR_NO_DEVICE, "hipSetDevice failed");
     return run_device_geometry(ctx, params, d_in, full_size, d_out, out_cap, hip_stream, 0, 0);
+}
+
+int ostft_run_device_noalloc(ostft_ctx* ctx, const ostft_params* params, const float* d_in, uint64_t full_size,
+                             float* d_out, uint64_t out_cap, void* hip_stream)
+{
+    if (!ctx)
+        return OSTFT_ERR_INVALID_ARG;
+    DeviceGuard dg(ctx->device);
+    if (!dg.ok)
+        return fail(ctx, OSTFT_ERR_NO_DEVICE, "hipSetDevice failed");
+    ostft::FusedArgs a;
+    int log_m = 0;
+    const int rc = ostft::ctx_prepare(ctx, params, d_in, full_size, d_out, out_cap, &a, &log_m, 0, 0, true);
+    if (rc != OSTFT_OK)
+        return rc;
+    hipStream_t s = static_cast<hipStream_t>(hip_stream);
+    if (a.frames == 0)
+        return OSTFT_OK;
+    if (ostft::fused_supported(log_m)) {
+        // the single kernel needs no scratch
+        hipError_t e = ostft::launch_fused(log_m, transform_of(params), a, 1, s);
+        return e == hipSuccess ? OSTFT_OK : hip_fail(ctx, e, "fused STFT launch");
+    }
+    return ostft::run_multipass_static(ctx, ctx->device, log_m, a, s);
 }
 
 int ostft_run_strided_device(ostft_ctx* ctx, const ostft_params* params, const float* d_in, uint64_t full_size,

@@ -26,6 +26,10 @@
 // in the frequency domain: subtracting the frame mean only changes the k = 0
 // terms.
 #include "../../include/occastft_hip.h"
+
+#include <mutex>
+#include <string>
+
 #include "batch.hpp"
 #include "epilogue.hpp"
 #include "multipass.hpp"
@@ -376,6 +380,116 @@ uint64_t multipass_chunk_frames(uint64_t budget, int log_m)
     return n ? n : 1;
 }
 
+namespace {
+
+// The twiddle tables of the two passes (resident in the context).  With
+// `noalloc` a table that is not resident yet is an error instead of being
+// built.
+struct PassTables {
+    const float2 *fine, *coarse, *r0, *r1, *r2r1;
+    uint32_t fine_bits;
+};
+
+int pass_tables(ostft_ctx* ctx, int log_m, const int* lr, bool noalloc, PassTables* t)
+{
+    int rc = OSTFT_OK;
+    const int64_t M = (int64_t)1 << log_m;
+    t->fine_bits = (uint32_t)(log_m + 1) / 2;
+    if (!(t->fine = ctx_twiddles(ctx, M, (int64_t)1 << t->fine_bits, &rc, noalloc)))
+        return rc;
+    if (!(t->coarse = ctx_twiddles(ctx, M >> t->fine_bits, M >> t->fine_bits, &rc, noalloc)))
+        return rc;
+    if (!(t->r0 = ctx_twiddles(ctx, (int64_t)1 << lr[0], (int64_t)1 << lr[0], &rc, noalloc)))
+        return rc;
+    if (!(t->r1 = ctx_twiddles(ctx, (int64_t)1 << lr[1], (int64_t)1 << lr[1], &rc, noalloc)))
+        return rc;
+    if (!(t->r2r1 = ctx_twiddles(ctx, (int64_t)2 << lr[1], (int64_t)1 << lr[1], &rc, noalloc)))
+        return rc;
+    return OSTFT_OK;
+}
+
+// The fewest chunks of at most `cap` frames, of equal size (a request of
+// 2^k + 1 frames gets no one-frame chunk at its end).
+uint64_t even_chunk(uint64_t frames, uint64_t cap)
+{
+    const uint64_t nchunks = (frames + cap - 1) / cap;
+    return nchunks ? (frames + nchunks - 1) / nchunks : 1;
+}
+
+// Both passes over the request's frames, `chunk` frames at a time through
+// `buf` (chunk * M complex values).
+int run_chunks(ostft_ctx* ctx, int log_m, const int* lr, const FusedArgs& a, const PassTables& t, float2* buf,
+               uint64_t chunk, hipStream_t s)
+{
+    PassArgs p;
+    p.sig = a.in;
+    p.L = a.L;
+    p.hop = a.hop;
+    p.win = a.win;
+    p.dst = buf;
+    p.tw_r = t.r0;
+    p.log_m = (uint32_t)log_m;
+
+    LastArgs la;
+    la.src = buf;
+    la.tw_r = t.r1;
+    la.tw_fine = t.fine;
+    la.tw_coarse = t.coarse;
+    la.tw_n = a.tw_n;
+    la.tw_2r = t.r2r1;
+    la.out = a.out;
+    la.out_limit = a.out_limit;
+    la.fine_bits = t.fine_bits;
+    la.log_m = (uint32_t)log_m;
+    la.combine = a.combine;
+    la.out_mode = a.out_mode;
+    la.dc = a.dc;
+
+    for (uint64_t f0 = 0; f0 < a.frames; f0 += chunk) {
+        const uint64_t f1 = a.frames - f0 < chunk ? a.frames : f0 + chunk;
+        p.frame0 = la.frame0 = f0;
+        p.frames = la.frames = f1;
+        hipError_t e = first_pass(lr[0], p, s);
+        if (e == hipSuccess)
+            e = last_pass(lr[1], la, s);
+        if (e != hipSuccess)
+            return ctx_hip_fail(ctx, e, "multi-pass launch");
+    }
+    return OSTFT_OK;
+}
+
+// Scratch of the allocation-free entry point: a fixed device array, one
+// per GPU, shared by every such call on that GPU (calls that overlap in
+// time on one GPU race on it).  Host code reaches it only through the
+// address hipGetSymbolAddress returns for the device (the array's name is a
+// host-side stand-in, not a device address).
+constexpr uint64_t STATIC_SCRATCH_BYTES = 64ull << 20;
+__device__ __align__(256) float2 scratch[STATIC_SCRATCH_BYTES / sizeof(float2)];
+
+constexpr int MAX_DEVICES = 64;
+std::mutex static_mu;
+float2* static_ptr[MAX_DEVICES] = {};
+
+}  // namespace
+
+uint64_t multipass_static_scratch_bytes() { return STATIC_SCRATCH_BYTES; }
+
+hipError_t multipass_static_scratch_init(int device)
+{
+    if (device < 0 || device >= MAX_DEVICES)
+        return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(static_mu);
+    if (static_ptr[device])
+        return hipSuccess;
+    // the caller made `device` current: this is that device's copy
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess)
+        return e;
+    static_ptr[device] = static_cast<float2*>(ptr);
+    return hipSuccess;
+}
+
 int run_multipass(ostft_ctx* ctx, int log_m, const FusedArgs& a, hipStream_t s)
 {
     int lr[4];
@@ -386,77 +500,52 @@ int run_multipass(ostft_ctx* ctx, int log_m, const FusedArgs& a, hipStream_t s)
     // chain (acquire / handoff) orders their GPU work across streams
     std::lock_guard<std::mutex> lk(ctx_scratch_mutex(ctx));
     Scratch& sc = ctx_scratch(ctx);
-    // the fewest chunks the budget allows, of equal size (a request of
-    // 2^k + 1 frames gets no one-frame chunk at its end)
-    const uint64_t cap = multipass_chunk_frames(sc.budget, log_m);
-    const uint64_t nchunks = (a.frames + cap - 1) / cap;
-    const uint64_t chunk = nchunks ? (a.frames + nchunks - 1) / nchunks : 1;
-    hipError_t e = sc.ensure((a.frames < chunk ? a.frames : chunk) * M);
-    if (e != hipSuccess)
-        return ctx_hip_fail(ctx, e, "multi-pass scratch allocation");
-    e = sc.acquire(s);
-    if (e != hipSuccess)
-        return ctx_hip_fail(ctx, e, "multi-pass scratch ordering");
-    int rc = OSTFT_OK;
-    const uint32_t fb = (uint32_t)(log_m + 1) / 2;
-    const float2* tw_fine = ctx_twiddles(ctx, (int64_t)M, (int64_t)1 << fb, &rc);
-    if (!tw_fine)
-        return rc;
-    const float2* tw_coarse = ctx_twiddles(ctx, (int64_t)(M >> fb), (int64_t)(M >> fb), &rc);
-    if (!tw_coarse)
-        return rc;
-    const float2* tw_r0 = ctx_twiddles(ctx, (int64_t)1 << lr[0], (int64_t)1 << lr[0], &rc);
-    if (!tw_r0)
-        return rc;
-    const float2* tw_r1 = ctx_twiddles(ctx, (int64_t)1 << lr[1], (int64_t)1 << lr[1], &rc);
-    if (!tw_r1)
-        return rc;
-    const float2* tw_2r1 = ctx_twiddles(ctx, (int64_t)2 << lr[1], (int64_t)1 << lr[1], &rc);
-    if (!tw_2r1)
-        return rc;
-
-    PassArgs p;
-    p.sig = a.in;
-    p.L = a.L;
-    p.hop = a.hop;
-    p.win = a.win;
-    p.dst = sc.a;
-    p.tw_r = tw_r0;
-    p.log_m = (uint32_t)log_m;
-
-    LastArgs la;
-    la.src = sc.a;
-    la.tw_r = tw_r1;
-    la.tw_fine = tw_fine;
-    la.tw_coarse = tw_coarse;
-    la.tw_n = a.tw_n;
-    la.tw_2r = tw_2r1;
-    la.out = a.out;
-    la.out_limit = a.out_limit;
-    la.fine_bits = fb;
-    la.log_m = (uint32_t)log_m;
-    la.combine = a.combine;
-    la.out_mode = a.out_mode;
-    la.dc = a.dc;
-
     // Frames go through both passes a chunk at a time, so the scratch
     // allocation stays bounded (1 GiB by default) whatever the signal
     // length.  Chunks small enough for their scratch to stay in the 256 MB
     // memory-side cache between the passes measured slower (64 MiB: +15%,
     // 128 MiB: +3% on windows 2^16..2^20): the passes are not bound by the
     // round trip's HBM traffic, and every chunk adds two launch tails.
-    for (uint64_t f0 = 0; f0 < a.frames; f0 += chunk) {
-        const uint64_t f1 = a.frames - f0 < chunk ? a.frames : f0 + chunk;
-        p.frame0 = la.frame0 = f0;
-        p.frames = la.frames = f1;
-        e = first_pass(lr[0], p, s);
-        if (e == hipSuccess)
-            e = last_pass(lr[1], la, s);
-        if (e != hipSuccess)
-            return ctx_hip_fail(ctx, e, "multi-pass launch");
-    }
+    const uint64_t chunk = even_chunk(a.frames, multipass_chunk_frames(sc.budget, log_m));
+    hipError_t e = sc.ensure((a.frames < chunk ? a.frames : chunk) * M);
+    if (e != hipSuccess)
+        return ctx_hip_fail(ctx, e, "multi-pass scratch allocation");
+    e = sc.acquire(s);
+    if (e != hipSuccess)
+        return ctx_hip_fail(ctx, e, "multi-pass scratch ordering");
+    PassTables t;
+    int rc = pass_tables(ctx, log_m, lr, false, &t);
+    if (rc == OSTFT_OK)
+        rc = run_chunks(ctx, log_m, lr, a, t, sc.a, chunk, s);
+    if (rc != OSTFT_OK)
+        return rc;
     e = sc.handoff(s);
     return e == hipSuccess ? OSTFT_OK : ctx_hip_fail(ctx, e, "multi-pass scratch event");
+}
+
+int run_multipass_static(ostft_ctx* ctx, int device, int log_m, const FusedArgs& a, hipStream_t s)
+{
+    int lr[4];
+    if (multipass_plan(log_m, lr) != 2)
+        return OSTFT_ERR_UNSUPPORTED;
+    const uint64_t frame_bytes = sizeof(float2) << log_m;
+    if (frame_bytes > STATIC_SCRATCH_BYTES)
+        return ctx_fail(ctx, OSTFT_ERR_TOO_LARGE,
+                        "one frame's intermediate spectrum (" + std::to_string(frame_bytes)
+                            + " B) exceeds the fixed scratch of the allocation-free entry point ("
+                            + std::to_string(STATIC_SCRATCH_BYTES) + " B)");
+    float2* buf = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(static_mu);
+        buf = device >= 0 && device < MAX_DEVICES ? static_ptr[device] : nullptr;
+    }
+    if (!buf)
+        return ctx_fail(ctx, OSTFT_ERR_HIP, "fixed scratch of this device was not set up at context creation");
+    PassTables t;
+    const int rc = pass_tables(ctx, log_m, lr, true, &t);
+    if (rc != OSTFT_OK)
+        return rc;
+    return run_chunks(ctx, log_m, lr, a, t, buf, even_chunk(a.frames, STATIC_SCRATCH_BYTES / frame_bytes), s);
 }
 
 }  // namespace ostft

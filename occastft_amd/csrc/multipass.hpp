@@ -28,12 +28,20 @@ struct Scratch {
 };
 
 // accessors into the context (defined in capi.hip)
-float2* ctx_twiddles(ostft_ctx* ctx, int64_t m, int64_t count, int* rc);
+// W_m^j, j < count; with `noalloc` only a table already resident (else null
+// and OSTFT_ERR_NOT_PREPARED in *rc)
+float2* ctx_twiddles(ostft_ctx* ctx, int64_t m, int64_t count, int* rc, bool noalloc = false);
 Scratch& ctx_scratch(ostft_ctx* ctx);
 std::mutex& ctx_scratch_mutex(ostft_ctx* ctx);  // held while a call queues its scratch launches
 
 // Runs the two passes (the split epilogue inside the last).  Returns an OSTFT_* status.
 int run_multipass(ostft_ctx* ctx, int log_m, const FusedArgs& a, hipStream_t s);
+// The same through the fixed per-device scratch array (no allocation, no
+// event; the tables must be resident).  Calls that overlap on one GPU race.
+int run_multipass_static(ostft_ctx* ctx, int device, int log_m, const FusedArgs& a, hipStream_t s);
+// Looks up the fixed scratch of the current device, `device` (once per device).
+hipError_t multipass_static_scratch_init(int device);
+uint64_t multipass_static_scratch_bytes();
 
 // Frames per chunk (both passes run a chunk before the next one starts):
 // as many as fit `budget` bytes of scratch, at least one.
