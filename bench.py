@@ -534,7 +534,7 @@ def bench_large(args, runner) -> dict:
     res["sweep"] = rows
     plan = {"exp": e, "ov": ov}
     counter_pats = (["first_pass_kernel", "last_pass_kernel"], 2)
-    cpu_in = x[: 1 << 24].cpu().numpy()
+    cpu_in = x[: 1 << 26].cpu().numpy()
     del x, out
     free_gpu()
     return res, plan, counter_pats, cpu_in
@@ -718,7 +718,7 @@ def bench_batch(args, runner):
     xf = {"fp32_tile_queue": None, "fp32_strided": 0, "mfma_fp16_strided": 1, "mfma_bf16_strided": 2}[best]
     pats = ["stft_batch"] if xf is None else [f"stft_fused_kernel<{dom_exp - 1}, {xf}>"]
     plan = {"form": best, "exp": dom_exp}
-    cpu_in = x[:64].cpu().numpy()
+    cpu_in = x[:256].cpu().numpy()
     del x, out
     free_gpu()
     return res, plan, (pats, 2), cpu_in
@@ -765,18 +765,37 @@ def make_trace(n: int, rate: float, seed: int, on_gpu: bool = False):
     rng = np.random.default_rng(seed)
     gaps = rng.exponential(1.0 / rate, n) if rate > 0 else np.zeros(n)
     arrivals = np.cumsum(gaps) - gaps[0]
-    g = torch.Generator(device="cuda").manual_seed(seed)
-    reqs = []
+    meta = []
     for i in range(n):
         L = int(np.exp(rng.uniform(np.log(2.0 ** 14), np.log(2.0 ** 22))))
         e = int(rng.choice(SERVE_EXPS))
         ov = float(rng.choice(SERVE_OVERLAPS))
         wopt = str(rng.choice(SERVE_WINDOWS))
-        f, a, ph = rng.uniform(0, 0.5), rng.uniform(0.1, 1.0), rng.uniform(0, 2 * np.pi)
-        t = torch.arange(L, dtype=torch.float64, device="cuda")
-        sig = (torch.rand(L, dtype=torch.float64, device="cuda", generator=g) * 2 - 1
-               + a * torch.sin(2 * np.pi * f * t + ph)).float()
-        reqs.append((float(arrivals[i]), sig if on_gpu else sig.cpu().numpy(),
+        meta.append((L, e, ov, wopt, rng.uniform(0, 0.5), rng.uniform(0.1, 1.0), rng.uniform(0, 2 * np.pi)))
+    # every signal in one buffer, generated a chunk at a time (a few kernels
+    # per chunk, not per request)
+    lens = np.array([m[0] for m in meta], np.int64)
+    padded = (lens + 15) // 16 * 16  # every signal starts 64-byte aligned
+    starts = np.concatenate([[0], np.cumsum(padded)[:-1]])
+    total = int(padded.sum())
+    buf = torch.empty(total, dtype=torch.float32, device="cuda")
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    st_d = torch.from_numpy(starts).cuda()
+    fr = torch.tensor([m[4] for m in meta], dtype=torch.float64, device="cuda")
+    am = torch.tensor([m[5] for m in meta], dtype=torch.float64, device="cuda")
+    ph = torch.tensor([m[6] for m in meta], dtype=torch.float64, device="cuda")
+    chunk = 1 << 25
+    for s in range(0, total, chunk):
+        idx = torch.arange(s, min(total, s + chunk), dtype=torch.int64, device="cuda")
+        req = torch.searchsorted(st_d, idx, right=True) - 1
+        t = (idx - st_d[req]).double()
+        noise = torch.rand(idx.numel(), dtype=torch.float32, device="cuda", generator=g) * 2 - 1
+        buf[s: s + idx.numel()] = noise + (am[req] * torch.sin(2 * np.pi * fr[req] * t + ph[req])).float()
+    del idx, req, t, noise
+    src = buf if on_gpu else buf.cpu().numpy()
+    reqs = []
+    for i, (L, e, ov, wopt, *_) in enumerate(meta):
+        reqs.append((float(arrivals[i]), src[starts[i]: starts[i] + L],
                      ostft.parse_options(wopt + " --power_return", e, ov), wopt))
     return reqs
 

@@ -16,7 +16,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libocca_stft_hip.so")
-SOURCES = ["capi.hip", "fused_small.hip", "multipass.hip", "tables.hip", "batch.hip", "server.hip"]
+SOURCES = ["capi.hip", "fused_small.hip", "multipass.hip", "tables.hip", "batch.hip", "server.hip",
+           "mfma_real_dft.hip"]
 ARCH = "gfx950"
 
 

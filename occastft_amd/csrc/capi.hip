@@ -53,6 +53,7 @@ struct ostft_ctx {
     std::mutex mu;                 // guards the table caches and `stream`
     std::map<std::pair<int64_t, int64_t>, float2*> twiddles;  // (m, count) -> W_m^j, j < count
     std::map<std::tuple<int, int64_t, uint32_t, int>, float*> windows;  // (kind, N, sigma bits, exact)
+    std::map<std::tuple<const float*, int, int>, void*> mfma_tables;  // (window table, log_m, format)
     std::mutex lanes_mu;
     std::condition_variable lanes_cv;
     std::vector<std::unique_ptr<Lane>> lanes;
@@ -309,6 +310,36 @@ float* get_window(ostft_ctx* ctx, int kind, int64_t n, float sigma, int exact, i
     return w;
 }
 
+// The matrix-core form's DFT-matrix fragments for (window table, size,
+// format), built on the GPU once and kept (with `noalloc`: looked up only).
+void* get_mfma_table(ostft_ctx* ctx, int log_m, int xf, const float* win, const float2* tw_n, int* rc,
+                     bool noalloc)
+{
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    const auto key = std::make_tuple(win, log_m, xf);
+    auto it = ctx->mfma_tables.find(key);
+    if (it != ctx->mfma_tables.end())
+        return it->second;
+    if (noalloc) {
+        *rc = not_prepared(ctx, "the matrix-core DFT table");
+        return nullptr;
+    }
+    void* t = nullptr;
+    hipError_t e = hipMalloc(&t, ostft::mfma_table_bytes(log_m));
+    if (e == hipSuccess)
+        e = ostft::launch_mfma_table(log_m, xf, win, tw_n, t, ctx->stream);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        if (t)
+            (void)hipFree(t);
+        *rc = hip_fail(ctx, e, "matrix-core DFT table");
+        return nullptr;
+    }
+    ctx->mfma_tables[key] = t;
+    return t;
+}
+
 }  // namespace
 
 namespace ostft {
@@ -382,6 +413,13 @@ int ctx_prepare(ostft_ctx* ctx, const ostft_params* params, const float* d_in, u
     if (fused_supported(plan.log_m)) {
         a->tw = get_twiddles(ctx, M, M, &rc, noalloc);
         if (!a->tw)
+            return rc;
+    }
+    a->mfma_b = nullptr;
+    const int xf = transform_of(params);
+    if (xf != XF_FP32 && mfma_gemm_supported(plan.log_m)) {
+        a->mfma_b = get_mfma_table(ctx, plan.log_m, xf, w, a->tw_n, &rc, noalloc);
+        if (!a->mfma_b)
             return rc;
     }
     *log_m = plan.log_m;
@@ -553,6 +591,8 @@ void ostft_destroy(ostft_ctx* ctx)
         for (auto& kv : ctx->twiddles)
             (void)hipFree(kv.second);
         for (auto& kv : ctx->windows)
+            (void)hipFree(kv.second);
+        for (auto& kv : ctx->mfma_tables)
             (void)hipFree(kv.second);
         for (auto& ln : ctx->lanes) {
             if (ln->d_in)

@@ -37,12 +37,20 @@ struct FusedArgs {
                           // fewer when hop > N, so a tile's span fits its staging LDS)
     uint64_t in_stride;   // strided batches: floats between signal y's input and y+1's
     uint64_t out_stride;  //                  and between their outputs (blockIdx.y = y)
+    const void* mfma_b;   // matrix-core form: its DFT-matrix fragment table (mfma_real_dft.hip)
 };
 
 // transform of the fused kernel: Stockham FFT in FP32, or the split-precision
 // MFMA DFT (windows 64, 256, 1024)
 enum Transform : int { XF_FP32 = 0, XF_MFMA_FP16 = 1, XF_MFMA_BF16 = 2 };
 bool mfma_supported(int log_m);
+// complex sizes whose matrix-core form is the GEMM real DFT of
+// mfma_real_dft.hip; its fragment table (per window table and format, built
+// once with launch_mfma_table, mfma_table_bytes bytes) goes in FusedArgs::mfma_b
+bool mfma_gemm_supported(int log_m);
+size_t mfma_table_bytes(int log_m);
+hipError_t launch_mfma_table(int log_m, int xf, const float* win, const float2* tw_n, void* table, hipStream_t s);
+hipError_t launch_mfma_gemm(int log_m, int xf, const FusedArgs& a, uint32_t count, hipStream_t s);
 
 // Single-kernel path: complex size M = 2^log_m, 5 <= log_m <= 14 (N = 64 .. 32768).
 bool fused_supported(int log_m);

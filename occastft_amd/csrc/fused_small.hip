@@ -28,7 +28,6 @@
 //   * Blocks are renumbered so each XCD walks a contiguous range of frames.
 #include "engine.hpp"
 #include "epilogue.hpp"
-#include "mfma_dft.hpp"
 #include "stockham.hpp"
 
 namespace ostft {
@@ -74,6 +73,7 @@ struct FusedArgsGlobal {
     uint64_t out_limit;
     int32_t combine, out_mode, dc, tile_frames;
     uint64_t in_stride, out_stride;
+    const void OSTFT_GLOBAL* mfma_b;
 };
 static_assert(sizeof(FusedArgsGlobal) == sizeof(FusedArgs), "FusedArgsGlobal mirrors FusedArgs");
 static_assert(offsetof(FusedArgsGlobal, out) == offsetof(FusedArgs, out), "FusedArgsGlobal mirrors FusedArgs");
@@ -99,6 +99,7 @@ __device__ __forceinline__ FusedArgs from_global(const FusedArgsGlobal& g)
     a.tile_frames = g.tile_frames;
     a.in_stride = g.in_stride;
     a.out_stride = g.out_stride;
+    a.mfma_b = (const void*)g.mfma_b;
     return a;
 }
 
@@ -116,7 +117,7 @@ __device__ __forceinline__ uint32_t spad(uint32_t q) { return q + ((q >> 7) << 2
 // (fewer at the end of the request; tile_frames <= G, below G only when the
 // hop exceeds the window, so that the tile's span fits the staging buffer).  Shared by the one-request kernel and the persistent
 // batch kernel, so a frame's result does not depend on which of them ran it.
-template <int LOG_M, int XF = XF_FP32>
+template <int LOG_M>
 __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t f0, float2* lds, float2* red,
                                              const uint32_t tid)
 {
@@ -202,8 +203,7 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
             wv[m] = w2[t + m * T];
     }
     PassTwiddles<LOG_M> ptw;
-    if constexpr (XF == XF_FP32)
-        load_pass_twiddles<LOG_M>(ptw, a.tw, t);
+    load_pass_twiddles<LOG_M>(ptw, a.tw, t);
     // epilogue of the N-value layouts (COMPAT_POWER, MAGNITUDE, HALF_COMPLEX):
     // a real frame's spectrum has X[N-k] = conj X[k], so bins k0 .. k0+B of
     // the packed spectrum give positions k0 .. k0+B-1 and M+k0 .. M+k0+B-1
@@ -341,11 +341,11 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
     }
 
     // ---- 3. DC removal of the windowed frame (DCRemove_Common) ----
-    // Windows below 2^OSTFT_FREQ_DC_LOG_M+1 (and the matrix-core forms)
+    // Windows below 2^OSTFT_FREQ_DC_LOG_M+1
     // subtract the frame mean here (sum by in-row DPP butterflies); larger
     // ones remove it after the FFT, where subtracting the mean from every
     // sample only changes bin 0.
-    constexpr bool FREQ_DC = XF == XF_FP32 && LOG_M >= OSTFT_FREQ_DC_LOG_M;
+    constexpr bool FREQ_DC = LOG_M >= OSTFT_FREQ_DC_LOG_M;
     if (!FREQ_DC && a.dc) {
         float2 s = make_float2(0.0f, 0.0f);
 #pragma unroll
@@ -359,7 +359,7 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
     }
 
     // ---- 4. FFT of the packed frame; Z stays in LDS, natural order ----
-    if constexpr (XF == XF_FP32) {
+    {
         stockham<LOG_M, 0>(v, lds + grp * PADM, ptw, t);
         if constexpr (FREQ_DC) {
             // z[n] - mu (1 + i) for all n, mu = (Re Z0 + Im Z0) / N, moves
@@ -373,13 +373,6 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
                 __syncthreads();
             }
         }
-    } else {
-        float2* z = lds + grp * PADM;
-#pragma unroll
-        for (int m = 0; m < 16; ++m)
-            z[pad(t + m * T)] = v[m];
-        __syncthreads();
-        mfma_fft<LOG_M, M, G, PADM, WG, XF>(lds, a.tw, tid);
     }
 
     // ---- 5. last radix-2 stage + output, the whole block's output at once ----
@@ -603,8 +596,13 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
     }
 }
 
-template <int LOG_M, int XF>
-__global__ void __launch_bounds__(Cfg<LOG_M>::WG) __attribute__((amdgpu_waves_per_eu(4)))
+// minimum waves per SIMD the one-request kernel is compiled for
+#ifndef OSTFT_FUSED_WAVES
+#define OSTFT_FUSED_WAVES 4
+#endif
+
+template <int LOG_M>
+__global__ void __launch_bounds__(Cfg<LOG_M>::WG) __attribute__((amdgpu_waves_per_eu(OSTFT_FUSED_WAVES)))
     stft_fused_kernel(const FusedArgs a)
 {
     using C = Cfg<LOG_M>;
@@ -613,7 +611,7 @@ __global__ void __launch_bounds__(Cfg<LOG_M>::WG) __attribute__((amdgpu_waves_pe
     FusedArgs ay = a;
     ay.in += (uint64_t)blockIdx.y * a.in_stride;
     ay.out += (uint64_t)blockIdx.y * a.out_stride;
-    fused_frames<LOG_M, XF>(ay, (uint64_t)xcd_block(blockIdx.x, gridDim.x) * (uint32_t)a.tile_frames, lds, red,
+    fused_frames<LOG_M>(ay, (uint64_t)xcd_block(blockIdx.x, gridDim.x) * (uint32_t)a.tile_frames, lds, red,
                             threadIdx.x);
 }
 
@@ -691,7 +689,7 @@ __global__ void __launch_bounds__(Cfg<LOG_M>::WG) __attribute__((amdgpu_waves_pe
     }
 }
 
-template <int LOG_M, int XF>
+template <int LOG_M>
 static hipError_t launch_t(const FusedArgs& a, uint32_t count, hipStream_t s)
 {
     using C = Cfg<LOG_M>;
@@ -702,22 +700,19 @@ static hipError_t launch_t(const FusedArgs& a, uint32_t count, hipStream_t s)
         return hipSuccess;
     if (blocks > 0x7fffffffull || count > 65535u)
         return hipErrorInvalidValue;
-    hipLaunchKernelGGL((stft_fused_kernel<LOG_M, XF>), dim3((uint32_t)blocks, count), dim3(C::WG), 0, s, a);
+    hipLaunchKernelGGL((stft_fused_kernel<LOG_M>), dim3((uint32_t)blocks, count), dim3(C::WG), 0, s, a);
     return hipGetLastError();
 }
 
 template <int LOG_M>
 static hipError_t launch_xf(int xf, const FusedArgs& a, uint32_t count, hipStream_t s)
 {
-    if constexpr (LOG_M == 5 || LOG_M == 7 || LOG_M == 9) {
-        if (xf == XF_MFMA_FP16)
-            return launch_t<LOG_M, XF_MFMA_FP16>(a, count, s);
-        if (xf == XF_MFMA_BF16)
-            return launch_t<LOG_M, XF_MFMA_BF16>(a, count, s);
-    }
+    // the matrix-core forms run the GEMM real DFT of mfma_real_dft.hip
+    if (xf != XF_FP32 && mfma_gemm_supported(LOG_M))
+        return launch_mfma_gemm(LOG_M, xf, a, count, s);
     if (xf != XF_FP32)
         return hipErrorInvalidValue;
-    return launch_t<LOG_M, XF_FP32>(a, count, s);
+    return launch_t<LOG_M>(a, count, s);
 }
 
 bool mfma_supported(int log_m) { return log_m == 5 || log_m == 7 || log_m == 9; }
