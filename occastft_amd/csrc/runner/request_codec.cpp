@@ -225,7 +225,8 @@ bool decode_request(const uint8_t* buf, size_t len, Request& out, std::string& e
             return false;
         }
         out.data.resize((size_t)d.count);
-        std::memcpy(out.data.data(), m.base[d.seg] + 8 * d.start, 4 * (size_t)d.count);
+        if (d.count)  // an empty list: nothing to copy (and no buffer to copy to)
+            std::memcpy(out.data.data(), m.base[d.seg] + 8 * d.start, 4 * (size_t)d.count);
     }
     return true;
 }

@@ -207,9 +207,9 @@ WsServer::WsServer(int port, std::string host, uint64_t max_message)
 WsServer::~WsServer()
 {
     stop();
-    for (auto& t : threads_)
-        if (t.joinable())
-            t.join();
+    for (auto& ct : threads_)
+        if (ct.t.joinable())
+            ct.t.join();
     if (lfd_ >= 0)
         ::close(lfd_);
 }
@@ -234,6 +234,9 @@ bool WsServer::listen(std::string& err)
         err = std::strerror(errno);
         return false;
     }
+    socklen_t al = sizeof(a);
+    if (port_ == 0 && ::getsockname(lfd_, reinterpret_cast<sockaddr*>(&a), &al) == 0)
+        port_ = ntohs(a.sin_port);  // port 0: the one the system chose
     return true;
 }
 
@@ -253,8 +256,23 @@ void WsServer::serve(Handler h)
         int one = 1;
         ::setsockopt(fd, IPPROTO_TCP, TCP_NODELAY, &one, sizeof(one));
         std::lock_guard<std::mutex> lk(cmu_);
-        threads_.emplace_back(&WsServer::connection, this, fd, h);
+        for (auto it = threads_.begin(); it != threads_.end();) {
+            if (it->done->load()) {
+                it->t.join();
+                it = threads_.erase(it);
+            } else {
+                ++it;
+            }
+        }
+        auto done = std::make_shared<std::atomic<bool>>(false);
+        threads_.push_back(ConnThread{std::thread(&WsServer::connection_thread, this, fd, h, done), done});
     }
+}
+
+void WsServer::connection_thread(int fd, Handler h, std::shared_ptr<std::atomic<bool>> done)
+{
+    connection(fd, std::move(h));
+    done->store(true);
 }
 
 void WsServer::stop()

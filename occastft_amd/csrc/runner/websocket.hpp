@@ -42,7 +42,7 @@ public:
     bool listen(std::string& err);
     void serve(Handler h);  // blocks until stop()
     void stop();            // safe from any thread, including a handler
-    int port() const { return port_; }
+    int port() const { return port_; }  // after listen(): the bound port (also for port 0)
 
 private:
     void connection(int fd, Handler h);
@@ -53,7 +53,14 @@ private:
     std::atomic<bool> stopping_{false};
     std::mutex cmu_;
     std::vector<std::shared_ptr<WsConnection>> conns_;
-    std::vector<std::thread> threads_;
+    // connection threads; a finished one (done set as its last step) is
+    // joined at the next accept, so a long-running server keeps only live ones
+    struct ConnThread {
+        std::thread t;
+        std::shared_ptr<std::atomic<bool>> done;
+    };
+    std::vector<ConnThread> threads_;
+    void connection_thread(int fd, Handler h, std::shared_ptr<std::atomic<bool>> done);
 };
 
 // exposed for tests
