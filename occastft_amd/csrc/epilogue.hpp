@@ -83,6 +83,22 @@ __device__ __forceinline__ float power_of(float2 x2) { return 0.25f * fmaf(x2.x,
 __device__ __forceinline__ float2 value_of(float2 x2) { return make_float2(0.5f * x2.x, 0.5f * x2.y); }
 __device__ __forceinline__ float2 conj_value_of(float2 x2) { return make_float2(0.5f * x2.x, -0.5f * x2.y); }
 
+// Stores of the multi-pass epilogue: plain by default (L2 merges the short
+// runs of neighbouring workgroups into whole lines before they reach HBM),
+// streaming with OSTFT_MP_NT=1.
+#ifndef OSTFT_MP_NT
+#define OSTFT_MP_NT 0
+#endif
+template <typename T>
+__device__ __forceinline__ void mp_store(T v, T* p)
+{
+#if OSTFT_MP_NT
+    nt_store(v, p);
+#else
+    *p = v;
+#endif
+}
+
 // Write the outputs of bin k of frame f (frame length N = 2M) from the
 // doubled pair b.
 //   COMPAT_POWER  sqrt(Re^2 + Re^2) at f*N + k and f*N + M + k
@@ -96,33 +112,33 @@ __device__ __forceinline__ void write_bin(float* __restrict__ out, int mode, uin
     const uint64_t base = f * (uint64_t)(2 * M);
     switch (mode) {
     case OUT_COMPAT_POWER:
-        nt_store(compat_power(b.lo), out + base + k);
-        nt_store(compat_power(b.hi), out + base + M + k);
+        mp_store(compat_power(b.lo), out + base + k);
+        mp_store(compat_power(b.hi), out + base + M + k);
         break;
     case OUT_MAGNITUDE:
-        nt_store(magnitude(b.lo), out + base + k);
-        nt_store(magnitude(b.hi), out + base + M + k);
+        mp_store(magnitude(b.lo), out + base + k);
+        mp_store(magnitude(b.hi), out + base + M + k);
         break;
     case OUT_HALF_COMPLEX: {
         float2* o2 = reinterpret_cast<float2*>(out);
         if (base + k < limit)
-            nt_store(value_of(b.lo), o2 + base + k);
+            mp_store(value_of(b.lo), o2 + base + k);
         if (base + M + k < limit)
-            nt_store(value_of(b.hi), o2 + base + M + k);
+            mp_store(value_of(b.hi), o2 + base + M + k);
         break;
     }
     case OUT_POWER: {
         float* o = out + f * (uint64_t)(M + 1);
-        nt_store(power_of(b.lo), o + k);
+        mp_store(power_of(b.lo), o + k);
         if (k == 0)
-            nt_store(power_of(b.hi), o + M);
+            mp_store(power_of(b.hi), o + M);
         break;
     }
     case OUT_HALF_SPECTRUM: {
         float2* o = reinterpret_cast<float2*>(out) + f * (uint64_t)(M + 1);
-        nt_store(value_of(b.lo), o + k);
+        mp_store(value_of(b.lo), o + k);
         if (k == 0)
-            nt_store(value_of(b.hi), o + M);
+            mp_store(value_of(b.hi), o + M);
         break;
     }
     default:

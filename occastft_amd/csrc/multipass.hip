@@ -103,6 +103,11 @@ void Scratch::destroy()
 #define OSTFT_MP_COL_PAD 1
 #endif
 
+// first pass: frames per workgroup-order group (see first_pass_kernel)
+#ifndef OSTFT_MP_FRAME_GROUP
+#define OSTFT_MP_FRAME_GROUP 1
+#endif
+
 constexpr int MP_THREADS = 512;
 constexpr int MP_ELEMS = 16 * MP_THREADS;  // complex values per workgroup
 
@@ -127,10 +132,22 @@ __global__ void __launch_bounds__(MP_THREADS) first_pass_kernel(const PassArgs a
     const uint64_t M = 1ull << a.log_m;
     const uint64_t bpg = M / MP_ELEMS;  // blocks per transform = (M / R) / CB
     const uint64_t blk = xcd_block(blockIdx.x, gridDim.x);
-    const uint64_t P = a.frame0 + blk / bpg;
-    if (P >= a.frames)
+    // Workgroup order: frames in groups of FG; within a group, consecutive
+    // workgroups (one XCD, about the same time) take the GB blocks of a block
+    // group for each frame of the group in turn, then the next block group.
+    // The GB blocks share their 128-byte lines (a block's rows are CB values
+    // long), overlapping frames of a group share rows and the group shares
+    // each window segment, so those come from HBM about once per group; FG = 1
+    // is plain frame order.
+    constexpr uint32_t GB = CB >= 16 ? 1u : 16u / CB;  // bpg is a multiple of GB (M >= 2^13)
+    const uint64_t F = a.frames - a.frame0, fg = blk / (OSTFT_MP_FRAME_GROUP * bpg);
+    const uint64_t p0 = fg * OSTFT_MP_FRAME_GROUP;
+    if (p0 >= F)
         return;  // uniform over the block
-    const uint32_t g0 = (uint32_t)(blk % bpg) * CB;
+    const uint64_t q = F - p0 < OSTFT_MP_FRAME_GROUP ? F - p0 : OSTFT_MP_FRAME_GROUP;  // frames in this group
+    const uint64_t i = blk - fg * OSTFT_MP_FRAME_GROUP * bpg, r = i % (q * GB);
+    const uint64_t P = a.frame0 + p0 + r / GB, bi = (i / (q * GB)) * GB + r % GB;
+    const uint32_t g0 = (uint32_t)bi * CB;
     const uint32_t stride = (uint32_t)(M >> LOG_R);
     PassTwiddles<LOG_R> tws;
     load_pass_twiddles<LOG_R>(tws, a.tw_r, tid % T);
