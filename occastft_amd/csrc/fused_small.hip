@@ -360,7 +360,7 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
 
     // ---- 4. FFT of the packed frame; Z stays in LDS, natural order ----
     {
-        stockham<LOG_M, 0>(v, lds + grp * PADM, ptw, t);
+        stockham<LOG_M, 0>(v, lds + grp * PADM, ptw, a.tw, t);
         if constexpr (FREQ_DC) {
             // z[n] - mu (1 + i) for all n, mu = (Re Z0 + Im Z0) / N, moves
             // Z0 to ((Re Z0 - Im Z0) / 2, (Im Z0 - Re Z0) / 2)
