@@ -404,7 +404,7 @@ def bench_sweep(args, runner):
     b = algorithmic_bytes(g, 1 << e)
     cold_ms = cold_time(lambda: runner.run_device(x, p, out=out, stream=stream), stream, max(3, args.steps))
     total_bytes = sum(algorithmic_bytes(g, 1 << e) for e, ov, name, p, g in reqs)
-    kernel_key = f"stft_fused_kernel<{e - 1}, 0> window {1 << e} overlap {ov} {name}"
+    kernel_key = f"stft_fused_kernel<{e - 1}> window {1 << e} overlap {ov} {name}"
     res = header(args, frames / (ms_per_step / 1e3), "frames/s", ms_per_step, {
         "workload": "plain fused FP32 small-window sweep: 1e8-sample signal (8 sinusoids + N(0,0.05^2)), "
                     "windows 2^6..2^12, overlap 0/50%, Hanning + DC removal, power and half-complex "
@@ -417,11 +417,10 @@ def bench_sweep(args, runner):
                                launch_ms_in_sweep=round(float(per_ms[dom]), 4))
     res["sweep"] = rows
     plan = {"exp": e, "ov": ov, "layout": name}
-    counter_pats = ([f"stft_fused_kernel<{e - 1}, 0>"], 2)
     cpu_in = x[: 1 << 26].cpu().numpy()  # the CPU sample is a prefix of this signal
     del x, out
     free_gpu()
-    return res, plan, counter_pats, kernel_key, cpu_in
+    return res, plan, kernel_key, cpu_in
 
 
 def cpu_sweep(runner, xs_full, target_s):
@@ -533,11 +532,10 @@ def bench_large(args, runner) -> dict:
                                launch_ms_in_sweep=round(float(per_ms[dom]), 4))
     res["sweep"] = rows
     plan = {"exp": e, "ov": ov}
-    counter_pats = (["first_pass_kernel", "last_pass_kernel"], 2)
     cpu_in = x[: 1 << 26].cpu().numpy()
     del x, out
     free_gpu()
-    return res, plan, counter_pats, cpu_in
+    return res, plan, cpu_in
 
 
 def cpu_large(xs_full, target_s):
@@ -715,13 +713,11 @@ def bench_batch(args, runner):
                                f"{best} window {dom_row['window']}, 4096 signals in one launch", b, cold_ms,
                                launch_ms_in_sweep=dom_row["ms"])
     res["sweep"] = rows
-    xf = {"fp32_tile_queue": None, "fp32_strided": 0, "mfma_fp16_strided": 1, "mfma_bf16_strided": 2}[best]
-    pats = ["stft_batch"] if xf is None else [f"stft_fused_kernel<{dom_exp - 1}, {xf}>"]
     plan = {"form": best, "exp": dom_exp}
     cpu_in = x[:256].cpu().numpy()
     del x, out
     free_gpu()
-    return res, plan, (pats, 2), cpu_in
+    return res, plan, cpu_in
 
 
 def cpu_batch(xs, target_s):
@@ -925,7 +921,7 @@ def bench_serve(args, runner):
     cpu_reqs = [(x, p, wopt) for _, x, p, wopt in reqs[:400]]
     del go
     free_gpu()
-    return res, {"rate": 0.0}, (["stft_batch"], 2), cpu_reqs
+    return res, {"rate": 0.0}, cpu_reqs
 
 
 def cpu_serve(cpu_reqs, target_s):
@@ -949,24 +945,38 @@ def cpu_serve(cpu_reqs, target_s):
 # ---------------------------------------------------------------- single-purpose modes
 
 def copy_probe(args) -> int:
-    """torch's device copy of 1 GiB: 2 GiB of HBM traffic per launch."""
+    """Achievable HBM rates of plain torch kernels (ceilings for this device):
+    a 1 GiB device copy (2 GiB of traffic; the counter calibration), a
+    write-only fill of 800 MB, and the 1:2 read:write mix of the sweep's
+    dominant request (400 MB read, each value written twice: 1.2 GB)."""
     import torch
 
     n = 1 << 28
     src = torch.ones(n, dtype=torch.float32, device="cuda")
     dst = torch.empty_like(src)
-    for _ in range(args.warmup):
-        dst.copy_(src)
-    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    s.record()
-    for _ in range(args.steps):
-        dst.copy_(src)
-    e.record()
-    torch.cuda.synchronize()
-    ms = s.elapsed_time(e) / args.steps
-    print(json.dumps({"copy_bytes_per_launch": 8 * n, "ms": round(ms, 4),
-                      "gbps": round(8 * n / (ms / 1e3) / 1e9, 1)}))
+
+    def rate(fn, nbytes):
+        for _ in range(args.warmup):
+            fn()
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        s.record()
+        for _ in range(args.steps):
+            fn()
+        e.record()
+        torch.cuda.synchronize()
+        ms = s.elapsed_time(e) / args.steps
+        return {"bytes": nbytes, "ms": round(ms, 4), "gbps": round(nbytes / (ms / 1e3) / 1e9, 1)}
+
+    res = {"copy_bytes_per_launch": 8 * n}
+    c = rate(lambda: dst.copy_(src), 8 * n)
+    res.update({"ms": c["ms"], "gbps": c["gbps"]})
+    m = 100_000_000
+    x = src[:m]
+    y = dst[: 2 * m]
+    res["write_only_fill"] = rate(lambda: y.fill_(1.0), 8 * m)
+    res["read1_write2"] = rate(lambda: y.view(-1, 2).copy_(x.view(-1, 1).expand(-1, 2)), 12 * m)
+    print(json.dumps(res))
     return 0
 
 
@@ -991,11 +1001,14 @@ def launch_only(args) -> int:
                     r.run_strided_device(x, p)
     else:
         x = make_signal(SIGNAL_LEN, SEED)
-        out = torch.empty(2 * SIGNAL_LEN + 65536, dtype=torch.float32, device="cuda")
+        reqs = []
         for spec in args.launch:
             e, ov, lay = spec.split(":")
             opts = "--hanning_window --remove_dc" + (" --half_complex_return" if lay == "hc" else "")
-            p = ostft.parse_options(opts, int(e), float(ov))
+            reqs.append(ostft.parse_options(opts, int(e), float(ov)))
+        need = max(ostft.geometry(p, SIGNAL_LEN)["out_floats"] for p in reqs)
+        out = torch.empty(need + 4, dtype=torch.float32, device="cuda")
+        for p in reqs:
             for _ in range(3):
                 r.run_device(x, p, out=out)
     torch.cuda.synchronize()
@@ -1054,23 +1067,23 @@ def main() -> int:
         return launch_only(args)
 
     runner = ostft.Runner(0)
-    plan, pats, subs = {}, {}, {}
+    plan, subs = {}, {}
     want = {"all": ("sweep", "large", "batch", "serve")}.get(args.workload, (args.workload,))
     cpu_inputs = {}
     c2_key = None
     if "sweep" in want:
         log("sweep (C2)")
-        res, plan["c2"], pats["c2"], c2_key, cpu_inputs["c2"] = bench_sweep(args, runner)
+        res, plan["c2"], c2_key, cpu_inputs["c2"] = bench_sweep(args, runner)
         subs["c2"] = res
     if "large" in want:
         log("large windows (C3)")
-        subs["c3"], plan["c3"], pats["c3"], cpu_inputs["c3"] = bench_large(args, runner)
+        subs["c3"], plan["c3"], cpu_inputs["c3"] = bench_large(args, runner)
     if "batch" in want:
         log("many signals (C4)")
-        subs["c4"], plan["c4"], pats["c4"], cpu_inputs["c4"] = bench_batch(args, runner)
+        subs["c4"], plan["c4"], cpu_inputs["c4"] = bench_batch(args, runner)
     if "serve" in want:
         log("serving (C5)")
-        subs["c5"], plan["c5"], pats["c5"], cpu_inputs["c5"] = bench_serve(args, runner)
+        subs["c5"], plan["c5"], cpu_inputs["c5"] = bench_serve(args, runner)
 
     if not args.no_cpu:
         for w, fn in (("c2", lambda: cpu_sweep(runner, cpu_inputs["c2"], args.cpu_seconds)),
