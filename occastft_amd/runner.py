@@ -56,7 +56,9 @@ def geometry(params: N.Params, full_size: int) -> dict:
 
 
 class Runner:
-    """One engine context on one HIP device (not reentrant: one per thread)."""
+    """One engine context on one HIP device.  It may be shared by several host
+    threads: host-buffer calls run concurrently on the context's staging
+    lanes, and `last_error` is the calling thread's."""
 
     def __init__(self, device: int = 0):
         L = N.lib()
