@@ -179,3 +179,85 @@ def test_config4_full_size(runner):
                 mx, _, zabs = frame_errors(got, ref, (1 << exp) + 2)
                 assert mx <= tol and zabs <= 1e-6, (exp, flags, r, mx)
             del out
+
+
+# ------------------------------------------------------------ edge geometry
+
+def _activate_geometry(runner, x, p, hop, frames):
+    import ctypes
+
+    per = {N.OUT_POWER: (1 << p.exp) // 2 + 1, N.OUT_HALF_SPECTRUM: (1 << p.exp) + 2}.get(p.output, 1 << p.exp)
+    out = np.empty(max(frames * per, 1), np.float32)
+    n = ctypes.c_uint64(0)
+    rc = N.lib().ostft_activate_geometry(runner._h, ctypes.byref(p), x.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                         x.size, hop, frames, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                         out.size, ctypes.byref(n))
+    assert rc == N.OK, runner.last_error
+    return out[: n.value]
+
+
+@pytest.mark.parametrize("exp", EXPS)
+@pytest.mark.parametrize("hop_of", ["1", "N/2+3", "N+1", "3N+5"])
+@pytest.mark.parametrize("flags", [N.FLAG_MFMA_FP16, N.FLAG_MFMA_BF16])
+def test_matrix_core_form_any_hop(runner, exp, hop_of, flags):
+    """The matrix-core kernels read frame f at f*hop for any hop: 1 (every
+    frame overlaps the next in all but one sample), odd, and beyond the
+    window (gaps), with a last frame running past the end of the signal and a
+    frame count that leaves the last tile partial."""
+    Nw = 1 << exp
+    hop = {"1": 1, "N/2+3": Nw // 2 + 3, "N+1": Nw + 1, "3N+5": 3 * Nw + 5}[hop_of]
+    rng = np.random.default_rng(exp * 13 + hop)
+    frames = 37  # not a multiple of any tile (16, 8 or 2 frames)
+    L = (frames - 1) * hop + Nw // 2
+    x = rng.standard_normal(L).astype(np.float32) + np.float32(0.3)
+    for out_mode in (N.OUT_POWER, N.OUT_HALF_SPECTRUM, N.OUT_COMPAT_POWER):
+        p = ostft.make_params(exp, 0.0, N.WIN_HANNING, 0.0, out_mode, N.FLAG_REMOVE_DC | EXACT | flags)
+        y = _activate_geometry(runner, x, p, hop, frames)
+        ref = po.stft64(x, exp, hop, frames, window=N.WIN_HANNING, remove_dc=True, out_mode=FP64_MODE[out_mode])
+        mx, _, zabs = frame_errors(y, np.asarray(ref, np.float64)[: y.size], per_frame(p))
+        assert mx <= 1e-5 and zabs <= 1e-6, (exp, hop, out_mode, flags, mx, zabs)
+
+
+@pytest.mark.parametrize("exp", EXPS)
+@pytest.mark.parametrize("case", ["shorter_than_window", "one_tile_and_a_frame", "misaligned_input"])
+def test_matrix_core_form_short_and_misaligned_signals(runner, exp, case):
+    """Signals shorter than one window (one zero-padded frame), a frame count
+    of one tile plus one, and a device input 4 bytes past a 16-byte boundary
+    (the window-64 kernel's vector loads fall back to scalar ones)."""
+    import torch
+
+    Nw = 1 << exp
+    tile = {6: 16, 8: 8, 10: 2}[exp]
+    rng = np.random.default_rng(exp + len(case))
+    L = {"shorter_than_window": Nw // 2 + 5, "one_tile_and_a_frame": tile * (Nw // 2) + Nw // 2,
+         "misaligned_input": 40 * Nw + 7}[case]
+    x = rng.uniform(-1, 1, L).astype(np.float32)
+    for out_mode in (N.OUT_HALF_COMPLEX, N.OUT_POWER):
+        p = ostft.make_params(exp, 0.5, N.WIN_BLACKMAN, 0.0, out_mode, N.FLAG_REMOVE_DC | EXACT | N.FLAG_MFMA_FP16)
+        if case == "misaligned_input":
+            buf = torch.zeros(L + 4, dtype=torch.float32, device="cuda")
+            buf[1: 1 + L] = torch.from_numpy(x).cuda()
+            xin = buf[1: 1 + L]
+            assert xin.data_ptr() % 16 == 4
+            y = runner.run_device(xin, p)
+            torch.cuda.synchronize()
+            y = y.cpu().numpy()
+        else:
+            y = run(runner, x, p)
+        r, g = fp64(x, p)
+        r = np.asarray(r, np.float64)[: y.size]
+        assert y.size == g["out_floats"] and g["frames"] >= 1
+        mx, _, zabs = frame_errors(y, r, per_frame(p))
+        assert mx <= 1e-5 and zabs <= 1e-6, (exp, case, out_mode, mx)
+
+
+def test_matrix_core_form_empty_signal(runner):
+    """An empty signal gives an empty result, as on the FP32 path."""
+    import torch
+
+    for exp in EXPS:
+        p = ostft.make_params(exp, 0.5, N.WIN_HANNING, 0.0, N.OUT_POWER, N.FLAG_MFMA_FP16)
+        g = ostft.geometry(p, 0)
+        y = runner.run_device(torch.zeros(0, dtype=torch.float32, device="cuda"), p)
+        torch.cuda.synchronize()
+        assert y.numel() == g["out_floats"]
