@@ -652,8 +652,7 @@ def bench_batch(args, runner):
     out = torch.empty(need, dtype=torch.float32, device="cuda")
     stream = torch.cuda.current_stream()
     xs_host = {r: x[r].cpu().numpy() for r in BATCH_CHECK_ROWS}
-    for e in BATCH_EXPS:
-        runner.autotune_batch(e, 1 << 27)
+    tuned = {str(1 << e): runner.autotune_batch(e, 1 << 27) for e in BATCH_EXPS}
     rows = []
     for form, _ in BATCH_FORMS:
         for e in BATCH_EXPS:
@@ -713,6 +712,7 @@ def bench_batch(args, runner):
                                f"{best} window {dom_row['window']}, 4096 signals in one launch", b, cold_ms,
                                launch_ms_in_sweep=dom_row["ms"])
     res["sweep"] = rows
+    res["autotune"] = {w: {"blocks_per_cu": c.blocks_per_cu, "shape": c.shape, "plan": c.plan} for w, c in tuned.items()}
     plan = {"form": best, "exp": dom_exp}
     cpu_in = x[:256].cpu().numpy()
     del x, out
@@ -889,9 +889,10 @@ def bench_serve(args, runner):
     st = stats[-1]
     span = st["span_s"]
     go, alg = resident_batch(runner, reqs)
+    choices = {}
     for e in SERVE_EXPS:
-        runner.autotune_batch(e, 1 << 27)
-    choices = {str(1 << e): runner.batch_config(e) for e in SERVE_EXPS}
+        c = runner.autotune_batch(e, 1 << 27)
+        choices[str(1 << e)] = {"blocks_per_cu": c.blocks_per_cu, "shape": c.shape, "plan": c.plan}
     go()
     torch.cuda.synchronize()
     stream = torch.cuda.current_stream()
@@ -914,7 +915,9 @@ def bench_serve(args, runner):
         "frames_per_s": round(cap["frames"] / cap["span_s"], 1) if cap["span_s"] > 0 else None,
         "p50_ms": round(cap["latency_p50_ms"], 3), "p99_ms": round(cap["latency_p99_ms"], 3),
         "mean_batch": round(cap["mean_batch"], 2), "failed": cap["failed"]}
-    res["autotune_persistent_blocks_per_cu"] = choices  # 0: one workgroup per tile
+    # blocks_per_cu 0: one workgroup per tile; shape: frames per workgroup
+    # (0: 256 threads, 1: 512, 2: 128); plan 1: the remainder radix first
+    res["autotune"] = choices
     res["roofline"] = roofline(alg / (cold_ms / 1e3) / 1e9,
                                "the 2000 requests resident in HBM as one batch (tile-queue launches per window "
                                "size + descriptor upload)", alg, cold_ms)

@@ -38,7 +38,7 @@
 extern "C" {
 #endif
 
-#define OSTFT_ABI_VERSION 1
+#define OSTFT_ABI_VERSION 2
 
 /* status codes */
 #define OSTFT_OK 0
@@ -211,21 +211,44 @@ typedef struct ostft_job {
  * `hip_stream`, no wait.  Outputs must not overlap. */
 int ostft_run_batch_device(ostft_ctx* ctx, const ostft_job* jobs, uint32_t njobs, void* hip_stream);
 
-/* Batch plan of window exponent `exp` (6..14): blocks_per_cu > 0 selects the
- * persistent form with that many workgroups per CU (clamped to the occupancy
- * limit), < 0 the one-workgroup-per-tile form (the default), 0 leaves it;
- * use_graphs 0/1 turns replay of captured launch sequences off/on (-1 leaves
- * it; exp 0 sets only that); *current receives the setting in force (0 for
- * one workgroup per tile). */
+/* Batch launch form of window exponent `exp` (6..15): blocks_per_cu > 0
+ * selects the persistent form with that many workgroups per CU (clamped to
+ * the occupancy limit; it also sets workgroup shape 0, the only one the
+ * persistent kernel is built for), < 0 the one-workgroup-per-tile form (the
+ * default), 0 leaves it; use_graphs 0/1 turns replay of captured launch
+ * sequences off/on (-1 leaves it; exp 0 sets only that); *current receives
+ * the setting in force (0 for one workgroup per tile). */
 int ostft_batch_config(ostft_ctx* ctx, int32_t exp, int32_t blocks_per_cu, int32_t use_graphs,
                        int32_t* current);
 
+/* Engine variant of window exponent `exp` (6..15) on this context.
+ *   plan   radix plan of the FP32 single-kernel transform: 0 runs the radix-16
+ *          passes first and the remainder radix (2, 4 or 8) last, 1 the
+ *          remainder first.  Exponents 9 and 13 have no remainder (plan 0
+ *          only).  Every FP32 request of that window on the context uses it
+ *          (ostft_run_device, _noalloc, _strided, batches), so batch results
+ *          stay bitwise equal to single requests; the two plans agree to
+ *          rounding, not bitwise.
+ *   shape  frames per workgroup of the batch kernels: 0 the default (256
+ *          threads, or one transform when it needs more), 1 512 threads
+ *          (exponents up to 13), 2 128 threads (up to 12).  Shapes 1 and 2
+ *          run in the one-workgroup-per-tile form.
+ * -1 leaves a setting; *cur_shape / *cur_plan (may be NULL) receive what is
+ * in force.  A plan or shape not built for `exp` is OSTFT_ERR_UNSUPPORTED.
+ * No counterpart in the reference, whose radix plan is fixed per size. */
+int ostft_engine_variant(ostft_ctx* ctx, int32_t exp, int32_t shape, int32_t plan, int32_t* cur_shape,
+                         int32_t* cur_plan);
+
 /* Times window exponent `exp` on a synthetic ragged batch of `samples`
- * samples in both forms (one workgroup per tile; persistent at every
- * workgroups-per-CU setting), keeps the fastest for this context and reports
- * it (*best_blocks_per_cu = 0 for one workgroup per tile). */
+ * samples under every built engine variant (radix plan x workgroup shape)
+ * and launch form (one workgroup per tile; persistent at every
+ * workgroups-per-CU setting, shape 0), keeps the fastest for this context
+ * and reports it (*best_blocks_per_cu = 0 for one workgroup per tile; any
+ * output pointer may be NULL).  The plan it keeps applies to every FP32
+ * request of that window on the context (see ostft_engine_variant); while it
+ * runs, other calls on the context keep the previous plan. */
 int ostft_autotune_batch(ostft_ctx* ctx, int32_t exp, uint64_t samples, int32_t* best_blocks_per_cu,
-                         double* best_ms);
+                         int32_t* best_shape, int32_t* best_plan, double* best_ms);
 
 /* Scratch budget of the two-pass path (windows 2^16..2^20), in bytes: its
  * frames go through both passes in chunks whose intermediate spectra fit the
@@ -254,7 +277,7 @@ typedef struct ostft_server_config {
     uint32_t max_batch;     /* requests per batch; 0: 512 */
     uint32_t max_inflight;  /* batches on the GPU at once; 0: 3 */
     int32_t use_graphs;     /* 1 (default when the config is NULL): replay captured launches */
-    int32_t autotune;       /* 1: time the persistent grid of every window size at creation */
+    int32_t autotune;       /* 1: autotune every batch window size (ostft_autotune_batch) at creation */
 } ostft_server_config;
 
 typedef struct ostft_server_stats {

@@ -24,6 +24,9 @@ ERR_TIMEOUT = 8
 ERR_SHUTDOWN = 9
 ERR_NOT_PREPARED = 10
 
+# OSTFT_ABI_VERSION of the header these bindings follow
+ABI_VERSION = 2
+
 # windows
 WIN_NONE, WIN_HANNING, WIN_HAMMING, WIN_BLACKMAN, WIN_NUTTALL = 0, 1, 2, 3, 4
 WIN_BLACKMAN_NUTTALL, WIN_BLACKMAN_HARRIS, WIN_FLATTOP, WIN_GAUSSIAN = 5, 6, 7, 8
@@ -57,6 +60,7 @@ EXPORTS = (
     "ostft_activate_geometry",
     "ostft_batch_config",
     "ostft_autotune_batch",
+    "ostft_engine_variant",
     "ostft_scratch_budget",
     "ostft_server_create",
     "ostft_server_destroy",
@@ -176,6 +180,9 @@ def lib() -> ctypes.CDLL:
     L.ostft_last_error.argtypes = [vp]
     L.ostft_abi_version.restype = ctypes.c_int
     L.ostft_abi_version.argtypes = []
+    if L.ostft_abi_version() != ABI_VERSION:
+        raise NativeMissing(f"{LIB_PATH} has ABI version {L.ostft_abi_version()}, these bindings {ABI_VERSION}: "
+                            "rebuild it with `python -m occastft_amd.build`")
     L.ostft_window_coefficients.restype = ctypes.c_int
     L.ostft_window_coefficients.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_float, fp]
     i32p = ctypes.POINTER(ctypes.c_int32)
@@ -191,7 +198,10 @@ def lib() -> ctypes.CDLL:
     L.ostft_batch_config.restype = ctypes.c_int
     L.ostft_batch_config.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, i32p]
     L.ostft_autotune_batch.restype = ctypes.c_int
-    L.ostft_autotune_batch.argtypes = [vp, ctypes.c_int32, ctypes.c_uint64, i32p, ctypes.POINTER(ctypes.c_double)]
+    L.ostft_autotune_batch.argtypes = [vp, ctypes.c_int32, ctypes.c_uint64, i32p, i32p, i32p,
+                                       ctypes.POINTER(ctypes.c_double)]
+    L.ostft_engine_variant.restype = ctypes.c_int
+    L.ostft_engine_variant.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, i32p, i32p]
     L.ostft_scratch_budget.restype = ctypes.c_int
     L.ostft_scratch_budget.argtypes = [vp, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)]
     L.ostft_server_create.restype = ctypes.c_int

@@ -1,8 +1,9 @@
 // batch.hip -- many requests of uneven length, window size and overlap in one
 // launch sequence: one launch per window-size class present over that class's
 // tile queue (tiles of G frames of one request), either one workgroup per tile
-// or a persistent grid pulling tiles from a device-side counter (measured per
-// class by the autotuner).
+// or a persistent grid pulling tiles from a device-side counter.  The launch
+// form, the workgroup shape (G) and the radix plan of each class are measured
+// by the autotuner.
 //
 // The descriptor block (per-class headers, the requests' FusedArgs, the
 // tile -> request table) is built on the host in pinned memory and uploaded
@@ -19,6 +20,8 @@
 #include <chrono>
 #include <cstring>
 #include <map>
+#include <string>
+#include <utility>
 #include <vector>
 
 #include "batch.hpp"
@@ -53,8 +56,11 @@ struct BatchState {
     int use_graphs = 1;
     int persistent[NCLS] = {};  // 1: persistent grid claiming tiles; 0: one workgroup per tile
     int per_cu[NCLS] = {};      // persistent workgroups per CU; 0: the occupancy limit
-    BatchClassInfo info[NCLS] = {};
-    bool info_ok[NCLS] = {};
+    int shape[NCLS] = {};       // workgroup shape (engine variant bits 1-2); persistent needs 0
+    int plan_override[NCLS];    // >= 0: radix plan for this class instead of the context's (autotuning)
+    int variant[NCLS] = {};     // engine variant of each class in the current batch
+    BatchClassInfo info[NCLS][FUSED_VARIANTS] = {};
+    bool info_ok[NCLS][FUSED_VARIANTS] = {};
     // per slot: the launches of every class, captured once; a batch sets
     // each class node's grid (its persistent grid, or one workgroup when the
     // class has no tiles) and replays it
@@ -63,8 +69,14 @@ struct BatchState {
     hipGraphNode_t node[2][NCLS] = {};
     uint32_t node_grid[2][NCLS] = {};
     int node_persistent[2][NCLS] = {};
+    int node_variant[2][NCLS] = {};
     hipStream_t capture = nullptr;
 
+    BatchState()
+    {
+        for (int& p : plan_override)
+            p = -1;
+    }
     void drop_graph(int i)
     {
         if (exec[i])
@@ -110,25 +122,28 @@ int state(ostft_ctx* ctx, BatchState** out)
     return OSTFT_OK;
 }
 
-int class_info(ostft_ctx* ctx, BatchState* st, int c)
+int class_info(ostft_ctx* ctx, BatchState* st, int c, int v)
 {
-    if (!st->info_ok[c]) {
-        const hipError_t e = batch_class_info(c + LOG_M_MIN, &st->info[c]);
+    if (!st->info_ok[c][v]) {
+        const hipError_t e = batch_class_info(c + LOG_M_MIN, v, &st->info[c][v]);
         if (e != hipSuccess)
             return ctx_hip_fail(ctx, e, "batch kernel occupancy");
-        if (st->info[c].max_blocks_per_cu < 1)
+        if (st->info[c][v].max_blocks_per_cu < 1)
             return ctx_fail(ctx, OSTFT_ERR_HIP, "batch kernel does not fit on a CU");
-        st->info_ok[c] = true;
+        st->info_ok[c][v] = true;
     }
     return OSTFT_OK;
 }
+
+// Engine variant of class c: its workgroup shape with radix plan `plan`.
+int class_variant(const BatchState* st, int c, int plan) { return st->shape[c] * 2 + plan; }
 
 // Grid of class c for a batch with `ntiles` tiles of it.
 uint32_t class_grid(const BatchState* st, int c, uint64_t ntiles)
 {
     if (!st->persistent[c])
         return (uint32_t)std::max<uint64_t>(ntiles, 1);
-    const int mx = st->info[c].max_blocks_per_cu;
+    const int mx = st->info[c][st->variant[c]].max_blocks_per_cu;
     const int pc = st->per_cu[c] > 0 ? std::min(st->per_cu[c], mx) : mx;
     return (uint32_t)pc * (uint32_t)st->cu_count;
 }
@@ -143,7 +158,7 @@ hipError_t launch_classes(BatchState* st, uint32_t mask, void* base, hipStream_t
         a.base = base;
         a.header_off = (uint64_t)c * sizeof(BatchClassHeader);
         const uint32_t grid = one_block ? 1u : class_grid(st, c, ntiles[c]);
-        const hipError_t e = launch_batch(c + LOG_M_MIN, a, grid, st->persistent[c] != 0, s);
+        const hipError_t e = launch_batch(c + LOG_M_MIN, st->variant[c], a, grid, st->persistent[c] != 0, s);
         if (e != hipSuccess)
             return e;
     }
@@ -183,7 +198,7 @@ hipError_t capture_slot(BatchState* st, int si)
             break;
         int c = -1;
         for (int k = 0; k < NCLS; ++k)
-            if (kp.func == batch_kernel_fn(k + LOG_M_MIN, st->persistent[k] != 0))
+            if (kp.func == batch_kernel_fn(k + LOG_M_MIN, st->variant[k], st->persistent[k] != 0))
                 c = k;
         if (c < 0 || st->node[si][c]) {
             e = hipErrorInvalidValue;
@@ -192,6 +207,7 @@ hipError_t capture_slot(BatchState* st, int si)
         st->node[si][c] = nodes[i];
         st->node_grid[si][c] = 1;
         st->node_persistent[si][c] = st->persistent[c];
+        st->node_variant[si][c] = st->variant[c];
     }
     hipGraphExec_t ex = nullptr;
     if (e == hipSuccess)
@@ -209,8 +225,8 @@ hipError_t capture_slot(BatchState* st, int si)
 hipError_t replay_slot(BatchState* st, int si, uint32_t mask, const uint64_t* ntiles, hipStream_t s)
 {
     for (int c = 0; st->exec[si] && c < NCLS; ++c)
-        if (st->node_persistent[si][c] != st->persistent[c]) {
-            // a class changed form: its node names another kernel
+        if (st->node_persistent[si][c] != st->persistent[c] || st->node_variant[si][c] != st->variant[c]) {
+            // a class changed form or variant: its node names another kernel
             (void)hipGraphExecDestroy(st->exec[si]);
             (void)hipGraphDestroy(st->graph[si]);
             st->exec[si] = nullptr;
@@ -292,6 +308,9 @@ int run_batch(ostft_ctx* ctx, const ostft_job* jobs, uint32_t njobs, hipStream_t
     uint64_t ntiles[NCLS] = {};
     std::vector<uint32_t> large, mfma;
     uint32_t mask = 0;
+    int plan[NCLS];  // radix plan of each class: the context's, read once per class per batch
+    for (int c = 0; c < NCLS; ++c)
+        plan[c] = -1;
     for (uint32_t i = 0; i < njobs; ++i) {
         if (args[i].frames == 0)
             continue;
@@ -307,10 +326,23 @@ int run_batch(ostft_ctx* ctx, const ostft_job* jobs, uint32_t njobs, hipStream_t
             continue;
         }
         const int c = lm[i] - LOG_M_MIN;
-        rc = class_info(ctx, st, c);
-        if (rc != OSTFT_OK)
-            return rc;
-        const uint64_t G = (uint64_t)args[i].tile_frames;  // <= the class's frames per workgroup
+        if (plan[c] < 0) {
+            plan[c] = st->plan_override[c] >= 0 ? st->plan_override[c] : args[i].plan;
+            if (!fused_plan_ok(lm[i], plan[c]))
+                plan[c] = 0;
+            st->variant[c] = class_variant(st, c, plan[c]);
+            if (!fused_variant_ok(lm[i], st->variant[c]))
+                return ctx_fail(ctx, OSTFT_ERR_INVALID_ARG, "batch variant not built");
+            rc = class_info(ctx, st, c, st->variant[c]);
+            if (rc != OSTFT_OK)
+                return rc;
+        }
+        // every request of a class runs the class's plan, in tiles of the
+        // variant's workgroup
+        args[i].plan = plan[c];
+        args[i].tile_frames = (int32_t)fused_tile_frames(
+            (uint64_t)st->info[c][st->variant[c]].frames_per_tile, 2ull << lm[i], args[i].hop);
+        const uint64_t G = (uint64_t)args[i].tile_frames;
         members[c].push_back(i);
         ntiles[c] += (args[i].frames + G - 1) / G;
         mask |= 1u << c;
@@ -395,6 +427,20 @@ int run_batch(ostft_ctx* ctx, const ostft_job* jobs, uint32_t njobs, hipStream_t
     return OSTFT_OK;
 }
 
+namespace {
+
+constexpr float AUTOTUNE_MARGIN = 0.02f;
+
+int class_of(ostft_ctx* ctx, int exp, int* c)
+{
+    *c = exp - 1 - LOG_M_MIN;
+    if (*c < 0 || *c >= NCLS)
+        return ctx_fail(ctx, OSTFT_ERR_UNSUPPORTED, "the batch kernels cover window exponents 6..15");
+    return OSTFT_OK;
+}
+
+}  // namespace
+
 int batch_config(ostft_ctx* ctx, int exp, int blocks_per_cu, int use_graphs, int* current)
 {
     BatchState* st = nullptr;
@@ -405,40 +451,104 @@ int batch_config(ostft_ctx* ctx, int exp, int blocks_per_cu, int use_graphs, int
         st->use_graphs = use_graphs ? 1 : 0;
     if (exp == 0)
         return OSTFT_OK;
-    const int c = exp - 1 - LOG_M_MIN;
-    if (c < 0 || c >= NCLS)
-        return ctx_fail(ctx, OSTFT_ERR_UNSUPPORTED, "the batch kernels cover window exponents 6..14");
-    rc = class_info(ctx, st, c);
+    int c = 0;
+    rc = class_of(ctx, exp, &c);
     if (rc != OSTFT_OK)
         return rc;
     // captured launches take a new setting at their next replay
     if (blocks_per_cu > 0) {
+        const int v = ctx_radix_plan(ctx, c + LOG_M_MIN);  // shape 0
+        rc = class_info(ctx, st, c, v);
+        if (rc != OSTFT_OK)
+            return rc;
+        st->shape[c] = 0;
         st->persistent[c] = 1;
-        st->per_cu[c] = std::min(blocks_per_cu, st->info[c].max_blocks_per_cu);
+        st->per_cu[c] = std::min(blocks_per_cu, st->info[c][v].max_blocks_per_cu);
     } else if (blocks_per_cu < 0) {
         st->persistent[c] = 0;
     }
-    if (current)
-        *current = st->persistent[c] ? (int)(class_grid(st, c, 0) / (uint32_t)st->cu_count) : 0;
+    if (current) {
+        int pc = 0;
+        if (st->persistent[c]) {
+            const int v = ctx_radix_plan(ctx, c + LOG_M_MIN);
+            rc = class_info(ctx, st, c, v);
+            if (rc != OSTFT_OK)
+                return rc;
+            const int mx = st->info[c][v].max_blocks_per_cu;
+            pc = st->per_cu[c] > 0 ? std::min(st->per_cu[c], mx) : mx;
+        }
+        *current = pc;
+    }
     return OSTFT_OK;
 }
 
-// Times one class over a synthetic ragged batch (`samples` samples in 32
-// requests, overlap 0.5, Hann, DC removal, compat power) in the dispatch form
-// and in the persistent form at each workgroups-per-CU setting, and keeps the
-// fastest (0 = dispatch).
-int autotune_batch(ostft_ctx* ctx, int exp, uint64_t samples, int* best_blocks_per_cu, double* best_ms)
+int engine_variant(ostft_ctx* ctx, int exp, int shape, int plan, int* cur_shape, int* cur_plan)
 {
     BatchState* st = nullptr;
     int rc = state(ctx, &st);
     if (rc != OSTFT_OK)
         return rc;
-    const int c = exp - 1 - LOG_M_MIN;
-    if (c < 0 || c >= NCLS)
-        return ctx_fail(ctx, OSTFT_ERR_UNSUPPORTED, "the batch kernels cover window exponents 6..14");
-    rc = class_info(ctx, st, c);
+    int c = 0;
+    rc = class_of(ctx, exp, &c);
     if (rc != OSTFT_OK)
         return rc;
+    const int log_m = c + LOG_M_MIN;
+    if (plan >= 0 && !fused_plan_ok(log_m, plan))
+        return ctx_fail(ctx, OSTFT_ERR_UNSUPPORTED, "radix plan " + std::to_string(plan) +
+                                                        " is not built for window exponent " + std::to_string(exp));
+    if (shape >= 0 && (shape > (FUSED_VARIANTS - 1) / 2 || !fused_variant_ok(log_m, shape * 2)))
+        return ctx_fail(ctx, OSTFT_ERR_UNSUPPORTED, "workgroup shape " + std::to_string(shape) +
+                                                        " is not built for window exponent " + std::to_string(exp));
+    if (plan >= 0)
+        ctx_set_radix_plan(ctx, log_m, plan);
+    if (shape >= 0) {
+        st->shape[c] = shape;
+        if (shape != 0)
+            st->persistent[c] = 0;  // the persistent kernel has shape 0 only
+    }
+    if (cur_shape)
+        *cur_shape = st->shape[c];
+    if (cur_plan)
+        *cur_plan = ctx_radix_plan(ctx, log_m);
+    return OSTFT_OK;
+}
+
+// Times one class over a synthetic ragged batch (`samples` samples in 32
+// requests, overlap 0.5, Hann, DC removal, compat power) under every built
+// variant (radix plan x workgroup shape) in the dispatch form, and for
+// shape 0 in the persistent form at each workgroups-per-CU setting.  The
+// default (variant 0, one workgroup per tile) is timed first and stays unless
+// a candidate beats it by more than AUTOTUNE_MARGIN: most candidates land
+// within a percent or two of each other, where one timing's noise would
+// otherwise decide.
+int autotune_batch(ostft_ctx* ctx, int exp, uint64_t samples, int* best_blocks_per_cu, int* best_shape,
+                   int* best_plan, double* best_ms)
+{
+    BatchState* st = nullptr;
+    int rc = state(ctx, &st);
+    if (rc != OSTFT_OK)
+        return rc;
+    int c = 0;
+    rc = class_of(ctx, exp, &c);
+    if (rc != OSTFT_OK)
+        return rc;
+    const int log_m = c + LOG_M_MIN;
+    // candidates: (variant, workgroups per CU); 0 per CU = one workgroup per tile
+    std::vector<std::pair<int, int>> cand;
+    for (int v = 0; v < FUSED_VARIANTS; ++v) {
+        if (!fused_variant_ok(log_m, v))
+            continue;
+        rc = class_info(ctx, st, c, v);
+        if (rc != OSTFT_OK)
+            return rc;
+        cand.push_back({v, 0});
+        if (v < 2) {  // shape 0: the persistent form too
+            const int mx = st->info[c][v].max_blocks_per_cu;
+            for (int k = 1; k < mx; k *= 2)
+                cand.push_back({v, k});
+            cand.push_back({v, mx});
+        }
+    }
     const uint64_t N = 1ull << exp;
     if (samples < N)
         samples = N;
@@ -493,22 +603,20 @@ int autotune_batch(ostft_ctx* ctx, int exp, uint64_t samples, int* best_blocks_p
             j.in = din + reinterpret_cast<uint64_t>(j.in);
             j.out = dout + reinterpret_cast<uint64_t>(j.out);
         }
-        const int saved_graphs = st->use_graphs, saved_pc = st->per_cu[c], saved_ps = st->persistent[c];
+        const int saved_graphs = st->use_graphs, saved_pc = st->per_cu[c], saved_ps = st->persistent[c],
+                  saved_shape = st->shape[c];
         st->use_graphs = 0;
-        const int mx = st->info[c].max_blocks_per_cu;
-        std::vector<int> cand{0};  // 0: one workgroup per tile; k > 0: persistent, k per CU
-        for (int k = 1; k < mx; k *= 2)
-            cand.push_back(k);
-        cand.push_back(mx);
-        int best = 0;
+        std::pair<int, int> best{0, 0};
         float best_t = 1e30f;
-        for (int k : cand) {
+        for (const auto& [v, k] : cand) {
+            st->shape[c] = v >> 1;
+            st->plan_override[c] = v & 1;
             st->persistent[c] = k > 0;
             st->per_cu[c] = k;
             // run; e0; run; e1: the second batch is prepared while the first
             // runs, so e0 -> e1 is the GPU's time for one batch
             float tmin = 1e30f;
-            for (int r = 0; r < 3 && rc == OSTFT_OK && e == hipSuccess; ++r) {
+            for (int r = 0; r < 5 && rc == OSTFT_OK && e == hipSuccess; ++r) {
                 rc = run_batch(ctx, jobs.data(), (uint32_t)jobs.size(), s);
                 if (rc == OSTFT_OK)
                     e = hipEventRecord(e0, s);
@@ -526,20 +634,29 @@ int autotune_batch(ostft_ctx* ctx, int exp, uint64_t samples, int* best_blocks_p
             }
             if (rc != OSTFT_OK || e != hipSuccess)
                 break;
-            if (tmin < best_t) {
+            // the first candidate is the default; later ones must win clearly
+            if (best_t == 1e30f || tmin < best_t * (1.0f - AUTOTUNE_MARGIN)) {
                 best_t = tmin;
-                best = k;
+                best = {v, k};
             }
         }
+        st->plan_override[c] = -1;
         st->use_graphs = saved_graphs;
         st->per_cu[c] = saved_pc;
         st->persistent[c] = saved_ps;
+        st->shape[c] = saved_shape;
         if (rc == OSTFT_OK && e == hipSuccess) {
             // captured launches take it at their next replay
-            st->persistent[c] = best > 0;
-            st->per_cu[c] = best;
+            st->shape[c] = best.first >> 1;
+            st->persistent[c] = best.second > 0;
+            st->per_cu[c] = best.second;
+            ctx_set_radix_plan(ctx, log_m, best.first & 1);
             if (best_blocks_per_cu)
-                *best_blocks_per_cu = best;
+                *best_blocks_per_cu = best.second;
+            if (best_shape)
+                *best_shape = best.first >> 1;
+            if (best_plan)
+                *best_plan = best.first & 1;
             if (best_ms)
                 *best_ms = best_t;
         }

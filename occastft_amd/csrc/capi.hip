@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cerrno>
 #include <cmath>
 #include <cstdlib>
@@ -63,6 +64,10 @@ struct ostft_ctx {
     ostft::BatchState* batch = nullptr;  // batch descriptors, graphs, plans (batch.hip)
     std::mutex err_mu;
     std::unordered_map<std::thread::id, std::string> errors;  // last error per calling thread
+    // radix plan of the single-kernel engine per complex size 2^log_m, read
+    // once per request: every path (one request, strided, batch, no-alloc)
+    // of one context computes a window size with the same plan
+    std::atomic<int> radix_plan[16] = {};
 };
 
 namespace {
@@ -354,6 +359,15 @@ BatchState*& ctx_batch(ostft_ctx* ctx) { return ctx->batch; }
 int ctx_device(const ostft_ctx* ctx) { return ctx->device; }
 int ctx_fail(ostft_ctx* ctx, int code, const std::string& msg) { return fail(ctx, code, msg); }
 int ctx_hip_fail(ostft_ctx* ctx, hipError_t e, const char* where) { return hip_fail(ctx, e, where); }
+int ctx_radix_plan(const ostft_ctx* ctx, int log_m)
+{
+    return log_m >= 0 && log_m < 16 ? ctx->radix_plan[log_m].load(std::memory_order_relaxed) : 0;
+}
+void ctx_set_radix_plan(ostft_ctx* ctx, int log_m, int plan)
+{
+    if (log_m >= 0 && log_m < 16)
+        ctx->radix_plan[log_m].store(plan, std::memory_order_relaxed);
+}
 
 // Validates one device-resident request and resolves its tables (window,
 // twiddles): everything a launch needs, in `a`.  The caller's device is current.
@@ -396,15 +410,15 @@ int ctx_prepare(ostft_ctx* ctx, const ostft_params* params, const float* d_in, u
     a->in_stride = 0;
     a->out_stride = 0;
     a->tile_frames = 1;
+    a->plan = 0;
+    a->reserved = 0;
     if (fused_supported(plan.log_m)) {
         // a tile of G frames stages G N samples; a hop beyond the window (only
         // reachable with an explicit geometry) takes fewer frames per tile, so
         // that (frames - 1) hop + N <= G N
-        const uint64_t G = (uint64_t)fused_group_frames(plan.log_m);
-        uint64_t tf = G;
-        if (plan.g.hop > N)
-            tf = std::min<uint64_t>(G, (G - 1) * N / plan.g.hop + 1);
-        a->tile_frames = (int32_t)tf;
+        a->tile_frames = (int32_t)fused_tile_frames((uint64_t)fused_group_frames(plan.log_m), (uint64_t)N,
+                                                    plan.g.hop);
+        a->plan = ctx_radix_plan(ctx, plan.log_m);
     }
     a->tw = nullptr;
     a->tw_n = get_twiddles(ctx, 2 * M, M, &rc, noalloc);  // W_N^k, k < M
@@ -739,8 +753,8 @@ int ostft_batch_config(ostft_ctx* ctx, int32_t exp, int32_t blocks_per_cu, int32
     return ostft::batch_config(ctx, exp, blocks_per_cu, use_graphs, current);
 }
 
-int ostft_autotune_batch(ostft_ctx* ctx, int32_t exp, uint64_t samples, int32_t* best_blocks_per_cu,
-                         double* best_ms)
+int ostft_engine_variant(ostft_ctx* ctx, int32_t exp, int32_t shape, int32_t plan, int32_t* cur_shape,
+                         int32_t* cur_plan)
 {
     if (!ctx)
         return OSTFT_ERR_INVALID_ARG;
@@ -748,7 +762,19 @@ int ostft_autotune_batch(ostft_ctx* ctx, int32_t exp, uint64_t samples, int32_t*
     if (!dg.ok)
         return fail(ctx, OSTFT_ERR_NO_DEVICE, "hipSetDevice failed");
     std::lock_guard<std::mutex> lk(ctx->batch_mu);
-    return ostft::autotune_batch(ctx, exp, samples, best_blocks_per_cu, best_ms);
+    return ostft::engine_variant(ctx, exp, shape, plan, cur_shape, cur_plan);
+}
+
+int ostft_autotune_batch(ostft_ctx* ctx, int32_t exp, uint64_t samples, int32_t* best_blocks_per_cu,
+                         int32_t* best_shape, int32_t* best_plan, double* best_ms)
+{
+    if (!ctx)
+        return OSTFT_ERR_INVALID_ARG;
+    DeviceGuard dg(ctx->device);
+    if (!dg.ok)
+        return fail(ctx, OSTFT_ERR_NO_DEVICE, "hipSetDevice failed");
+    std::lock_guard<std::mutex> lk(ctx->batch_mu);
+    return ostft::autotune_batch(ctx, exp, samples, best_blocks_per_cu, best_shape, best_plan, best_ms);
 }
 
 int ostft_scratch_budget(ostft_ctx* ctx, uint64_t bytes, uint64_t* current)

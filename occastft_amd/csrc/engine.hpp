@@ -38,6 +38,8 @@ struct FusedArgs {
     uint64_t in_stride;   // strided batches: floats between signal y's input and y+1's
     uint64_t out_stride;  //                  and between their outputs (blockIdx.y = y)
     const void* mfma_b;   // matrix-core form: its DFT-matrix fragment table (mfma_real_dft.hip)
+    int32_t plan;         // radix plan of the FP32 engine (0 or 1, see Cfg in stockham.hpp)
+    int32_t reserved;
 };
 
 // transform of the fused kernel: Stockham FFT in FP32, or the split-precision
@@ -56,6 +58,18 @@ hipError_t launch_mfma_gemm(int log_m, int xf, const FusedArgs& a, uint32_t coun
 bool fused_supported(int log_m);
 // frames per workgroup (G) of the single kernel at complex size 2^log_m
 int fused_group_frames(int log_m);
+// frames per tile for a workgroup of G frames, window N and hop: G, fewer when
+// the hop exceeds the window, so that the tile's span (tf - 1) hop + N fits
+// the G N samples of the staging buffer
+uint64_t fused_tile_frames(uint64_t G, uint64_t N, uint64_t hop);
+// Variants of the FP32 engine (Cfg<LOG_M, V> in stockham.hpp): bit 0 the
+// radix plan (1: the remainder radix first), bits 1-2 the workgroup shape of
+// the batch kernels (0: 256 threads or one transform, 1: 512, 2: 128).  Only
+// variants that differ from variant 0 are built: a plan of 1 where the size
+// has a remainder radix, shapes that change the workgroup.
+constexpr int FUSED_VARIANTS = 6;
+bool fused_variant_ok(int log_m, int variant);
+bool fused_plan_ok(int log_m, int plan);
 // `count` signals (grid.y), signal y at a.in + y in_stride / a.out + y out_stride
 hipError_t launch_fused(int log_m, int xf, const FusedArgs& a, uint32_t count, hipStream_t s);
 
@@ -82,11 +96,13 @@ struct BatchClassInfo {
     int threads;            // workgroup size
     int max_blocks_per_cu;  // occupancy limit of the persistent batch kernel
 };
-// persistent: a fixed grid claiming tiles from the class counter; otherwise
-// one workgroup per tile (grid = the class's tile count)
-hipError_t launch_batch(int log_m, const BatchClassArgs& b, uint32_t grid, bool persistent, hipStream_t s);
-hipError_t batch_class_info(int log_m, BatchClassInfo* info);
-const void* batch_kernel_fn(int log_m, bool persistent);  // the kernel's host handle (graph nodes name it)
+// persistent: a fixed grid claiming tiles from the class counter (workgroup
+// shape 0 only); otherwise one workgroup per tile (grid = the class's tile count)
+hipError_t launch_batch(int log_m, int variant, const BatchClassArgs& b, uint32_t grid, bool persistent,
+                        hipStream_t s);
+hipError_t batch_class_info(int log_m, int variant, BatchClassInfo* info);
+// the kernel's host handle (graph nodes name it)
+const void* batch_kernel_fn(int log_m, int variant, bool persistent);
 
 // FP64 table generators (run once per table).
 // tw[j] = exp(-2 pi i j / m) for j < count (count <= m)

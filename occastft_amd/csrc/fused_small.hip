@@ -74,6 +74,7 @@ struct FusedArgsGlobal {
     int32_t combine, out_mode, dc, tile_frames;
     uint64_t in_stride, out_stride;
     const void OSTFT_GLOBAL* mfma_b;
+    int32_t plan, reserved;
 };
 static_assert(sizeof(FusedArgsGlobal) == sizeof(FusedArgs), "FusedArgsGlobal mirrors FusedArgs");
 static_assert(offsetof(FusedArgsGlobal, out) == offsetof(FusedArgs, out), "FusedArgsGlobal mirrors FusedArgs");
@@ -100,6 +101,8 @@ __device__ __forceinline__ FusedArgs from_global(const FusedArgsGlobal& g)
     a.in_stride = g.in_stride;
     a.out_stride = g.out_stride;
     a.mfma_b = (const void*)g.mfma_b;
+    a.plan = g.plan;
+    a.reserved = 0;
     return a;
 }
 
@@ -117,11 +120,11 @@ __device__ __forceinline__ uint32_t spad(uint32_t q) { return q + ((q >> 7) << 2
 // (fewer at the end of the request; tile_frames <= G, below G only when the
 // hop exceeds the window, so that the tile's span fits the staging buffer).  Shared by the one-request kernel and the persistent
 // batch kernel, so a frame's result does not depend on which of them ran it.
-template <int LOG_M>
+template <int LOG_M, int V = 0>
 __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t f0, float2* lds, float2* red,
                                              const uint32_t tid)
 {
-    using C = Cfg<LOG_M>;
+    using C = Cfg<LOG_M, V>;
     constexpr int M = C::M, N = 2 * M, T = C::T, G = C::G, WG = C::WG, PADM = C::PADM;
     float* sf = reinterpret_cast<float*>(lds);
 
@@ -202,8 +205,8 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
         for (int m = 0; m < 16; ++m)
             wv[m] = w2[t + m * T];
     }
-    PassTwiddles<LOG_M> ptw;
-    load_pass_twiddles<LOG_M>(ptw, a.tw, t);
+    PassTwiddles<LOG_M, V> ptw;
+    load_pass_twiddles<LOG_M, V>(ptw, a.tw, t);
     // epilogue of the N-value layouts (COMPAT_POWER, MAGNITUDE, HALF_COMPLEX):
     // a real frame's spectrum has X[N-k] = conj X[k], so bins k0 .. k0+B of
     // the packed spectrum give positions k0 .. k0+B-1 and M+k0 .. M+k0+B-1
@@ -360,7 +363,8 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
 
     // ---- 4. FFT of the packed frame; Z stays in LDS, natural order ----
     {
-        stockham<LOG_M, 0>(v, lds + grp * PADM, ptw, a.tw, t);
+        first_pass_order<LOG_M, V>(v);
+        stockham<LOG_M, 0, V>(v, lds + grp * PADM, ptw, t);
         if constexpr (FREQ_DC) {
             // z[n] - mu (1 + i) for all n, mu = (Re Z0 + Im Z0) / N, moves
             // Z0 to ((Re Z0 - Im Z0) / 2, (Im Z0 - Re Z0) / 2)
@@ -601,18 +605,19 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
 #define OSTFT_FUSED_WAVES 4
 #endif
 
-template <int LOG_M>
-__global__ void __launch_bounds__(Cfg<LOG_M>::WG) __attribute__((amdgpu_waves_per_eu(OSTFT_FUSED_WAVES)))
+// the one-request kernel, radix plan PLAN (workgroup shape 0)
+template <int LOG_M, int PLAN>
+__global__ void __launch_bounds__((Cfg<LOG_M, PLAN>::WG)) __attribute__((amdgpu_waves_per_eu(OSTFT_FUSED_WAVES)))
     stft_fused_kernel(const FusedArgs a)
 {
-    using C = Cfg<LOG_M>;
+    using C = Cfg<LOG_M, PLAN>;
     __shared__ float2 lds[C::G * C::PADM];
     __shared__ float2 red[C::WG / 64];
     FusedArgs ay = a;
     ay.in += (uint64_t)blockIdx.y * a.in_stride;
     ay.out += (uint64_t)blockIdx.y * a.out_stride;
-    fused_frames<LOG_M>(ay, (uint64_t)xcd_block(blockIdx.x, gridDim.x) * (uint32_t)a.tile_frames, lds, red,
-                            threadIdx.x);
+    fused_frames<LOG_M, PLAN>(ay, (uint64_t)xcd_block(blockIdx.x, gridDim.x) * (uint32_t)a.tile_frames, lds, red,
+                              threadIdx.x);
 }
 
 // Batch of requests of one window size, two forms over the same tile queue
@@ -627,11 +632,11 @@ __global__ void __launch_bounds__(Cfg<LOG_M>::WG) __attribute__((amdgpu_waves_pe
 // of its previous tile's stores; a dispatched workgroup simply ends and leaves
 // its stores to drain.  Which form is faster for a window size is measured
 // (ostft_autotune_batch).
-template <int LOG_M>
-__global__ void __launch_bounds__(Cfg<LOG_M>::WG) __attribute__((amdgpu_waves_per_eu(4)))
+template <int LOG_M, int V>
+__global__ void __launch_bounds__((Cfg<LOG_M, V>::WG)) __attribute__((amdgpu_waves_per_eu(4)))
     stft_batch_dispatch_kernel(const char* __restrict__ blk, const uint32_t header_off)
 {
-    using C = Cfg<LOG_M>;
+    using C = Cfg<LOG_M, V>;
     __shared__ float2 lds[C::G * C::PADM];
     __shared__ float2 red[C::WG / 64];
     const BatchClassHeader* h = reinterpret_cast<const BatchClassHeader*>(blk + header_off);
@@ -643,16 +648,18 @@ __global__ void __launch_bounds__(Cfg<LOG_M>::WG) __attribute__((amdgpu_waves_pe
     const BatchJob* jobs = reinterpret_cast<const BatchJob*>(blk + h->jobs_off);
     const uint32_t j = reinterpret_cast<const uint32_t*>(blk + h->tiles_off)[tile];
     const FusedArgs a = from_global(*reinterpret_cast<const FusedArgsGlobal*>(&jobs[j].a));
-    fused_frames<LOG_M>(a, (uint64_t)(tile - jobs[j].tile_begin) * (uint32_t)a.tile_frames, lds, red, threadIdx.x);
+    fused_frames<LOG_M, V>(a, (uint64_t)(tile - jobs[j].tile_begin) * (uint32_t)a.tile_frames, lds, red,
+                           threadIdx.x);
 }
 
-template <int LOG_M>
-__global__ void __launch_bounds__(Cfg<LOG_M>::WG) __attribute__((amdgpu_waves_per_eu(BatchWaves<LOG_M>::value)))
+template <int LOG_M, int PLAN>
+__global__ void __launch_bounds__((Cfg<LOG_M, PLAN>::WG))
+    __attribute__((amdgpu_waves_per_eu(BatchWaves<LOG_M>::value)))
     stft_batch_kernel(const char* __restrict__ blk, uint32_t* __restrict__ counter, const uint32_t header_off)
 {
     // `blk` is read-only here (the counter is reached only through `counter`),
     // which lets the compiler read the descriptors with scalar loads
-    using C = Cfg<LOG_M>;
+    using C = Cfg<LOG_M, PLAN>;
     __shared__ float2 lds[C::G * C::PADM];
     __shared__ float2 red[C::WG / 64];
     __shared__ uint32_t claim;
@@ -680,7 +687,7 @@ __global__ void __launch_bounds__(Cfg<LOG_M>::WG) __attribute__((amdgpu_waves_pe
         // kernel's budget)
         uint32_t tid = threadIdx.x;
         asm volatile("" : "+v"(tid));
-        fused_frames<LOG_M>(a, (uint64_t)(tile - jobs[j].tile_begin) * (uint32_t)a.tile_frames, lds, red, tid);
+        fused_frames<LOG_M, PLAN>(a, (uint64_t)(tile - jobs[j].tile_begin) * (uint32_t)a.tile_frames, lds, red, tid);
         // every thread read `claim` before the body's first barrier, so it
         // can be overwritten now; the barrier publishes it and frees the LDS
         if (threadIdx.x == 0)
@@ -689,18 +696,61 @@ __global__ void __launch_bounds__(Cfg<LOG_M>::WG) __attribute__((amdgpu_waves_pe
     }
 }
 
+// host-side compile-time loop: f(std::integral_constant<int, i>) for i in [B, E)
+template <int B, int E, typename F>
+static void host_for(F&& f)
+{
+    if constexpr (B < E) {
+        f(std::integral_constant<int, B>{});
+        host_for<B + 1, E>(f);
+    }
+}
+
+// variant validity (see FUSED_VARIANTS in engine.hpp): a plan of 1 only where
+// the size has a remainder radix; workgroup shapes only where they change the
+// workgroup (512 threads while a transform has fewer, 128 while it has at
+// most 128)
+template <int LOG_M, int V>
+constexpr bool variant_ok()
+{
+    constexpr int PLAN = V & 1, SHAPE = V >> 1;
+    return V >= 0 && V < FUSED_VARIANTS && !(PLAN && LOG_M % 4 == 0) && !(SHAPE == 1 && LOG_M > 12) &&
+           !(SHAPE == 2 && LOG_M > 11);
+}
+
+// f(integral_constant<int, V>) for the runtime variant v, if it is built
+template <int LOG_M, typename F>
+static hipError_t with_variant(int v, F&& f)
+{
+    hipError_t e = hipErrorInvalidValue;
+    host_for<0, FUSED_VARIANTS>([&](auto vc) {
+        constexpr int V = decltype(vc)::value;
+        if constexpr (variant_ok<LOG_M, V>()) {
+            if (v == V)
+                e = f(vc);
+        }
+    });
+    return e;
+}
+
 template <int LOG_M>
 static hipError_t launch_t(const FusedArgs& a, uint32_t count, hipStream_t s)
 {
-    using C = Cfg<LOG_M>;
-    if (a.tile_frames < 1 || a.tile_frames > C::G)
+    using C = Cfg<LOG_M>;  // both plans share the workgroup
+    if (a.tile_frames < 1 || a.tile_frames > C::G || (a.plan != 0 && a.plan != 1))
         return hipErrorInvalidValue;
     const uint64_t blocks = (a.frames + a.tile_frames - 1) / a.tile_frames;
     if (blocks == 0 || count == 0)
         return hipSuccess;
     if (blocks > 0x7fffffffull || count > 65535u)
         return hipErrorInvalidValue;
-    hipLaunchKernelGGL((stft_fused_kernel<LOG_M>), dim3((uint32_t)blocks, count), dim3(C::WG), 0, s, a);
+    if constexpr (variant_ok<LOG_M, 1>()) {
+        if (a.plan == 1) {
+            hipLaunchKernelGGL((stft_fused_kernel<LOG_M, 1>), dim3((uint32_t)blocks, count), dim3(C::WG), 0, s, a);
+            return hipGetLastError();
+        }
+    }
+    hipLaunchKernelGGL((stft_fused_kernel<LOG_M, 0>), dim3((uint32_t)blocks, count), dim3(C::WG), 0, s, a);
     return hipGetLastError();
 }
 
@@ -718,126 +768,131 @@ static hipError_t launch_xf(int xf, const FusedArgs& a, uint32_t count, hipStrea
 bool mfma_supported(int log_m) { return log_m == 5 || log_m == 7 || log_m == 9; }
 
 template <int LOG_M>
-static hipError_t launch_batch_t(const BatchClassArgs& b, uint32_t grid, bool persistent, hipStream_t s)
+static hipError_t launch_batch_t(int v, const BatchClassArgs& b, uint32_t grid, bool persistent, hipStream_t s)
 {
     if (grid == 0)
         return hipErrorInvalidValue;
     char* blk = static_cast<char*>(b.base);
-    if (persistent) {
-        uint32_t* counter = reinterpret_cast<uint32_t*>(blk + b.header_off + offsetof(BatchClassHeader, counter));
-        hipLaunchKernelGGL(stft_batch_kernel<LOG_M>, dim3(grid), dim3(Cfg<LOG_M>::WG), 0, s,
-                           static_cast<const char*>(blk), counter, (uint32_t)b.header_off);
-    } else {
-        hipLaunchKernelGGL(stft_batch_dispatch_kernel<LOG_M>, dim3(grid), dim3(Cfg<LOG_M>::WG), 0, s,
-                           static_cast<const char*>(blk), (uint32_t)b.header_off);
-    }
-    return hipGetLastError();
+    return with_variant<LOG_M>(v, [&](auto vc) -> hipError_t {
+        constexpr int V = decltype(vc)::value;
+        using C = Cfg<LOG_M, V>;
+        if (persistent) {
+            if constexpr (C::SHAPE == 0) {
+                uint32_t* counter =
+                    reinterpret_cast<uint32_t*>(blk + b.header_off + offsetof(BatchClassHeader, counter));
+                hipLaunchKernelGGL((stft_batch_kernel<LOG_M, C::PLAN>), dim3(grid), dim3(C::WG), 0, s,
+                                   static_cast<const char*>(blk), counter, (uint32_t)b.header_off);
+            } else {
+                return hipErrorInvalidValue;  // the persistent form has workgroup shape 0 only
+            }
+        } else {
+            hipLaunchKernelGGL((stft_batch_dispatch_kernel<LOG_M, V>), dim3(grid), dim3(C::WG), 0, s,
+                               static_cast<const char*>(blk), (uint32_t)b.header_off);
+        }
+        return hipGetLastError();
+    });
 }
 
 template <int LOG_M>
-static hipError_t batch_info_t(BatchClassInfo* info)
+static hipError_t batch_info_t(int v, BatchClassInfo* info)
 {
-    info->frames_per_tile = Cfg<LOG_M>::G;
-    info->threads = Cfg<LOG_M>::WG;
-    int per_cu = 0;
-    const hipError_t e =
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, stft_batch_kernel<LOG_M>, Cfg<LOG_M>::WG, 0);
-    info->max_blocks_per_cu = per_cu;
-    return e;
+    return with_variant<LOG_M>(v, [&](auto vc) -> hipError_t {
+        constexpr int V = decltype(vc)::value;
+        using C = Cfg<LOG_M, V>;
+        info->frames_per_tile = C::G;
+        info->threads = C::WG;
+        int per_cu = 0;
+        hipError_t e;
+        if constexpr (C::SHAPE == 0)
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, stft_batch_kernel<LOG_M, C::PLAN>, C::WG, 0);
+        else
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, stft_batch_dispatch_kernel<LOG_M, V>, C::WG,
+                                                             0);
+        info->max_blocks_per_cu = per_cu;
+        return e;
+    });
 }
 
 template <int LOG_M>
-static const void* batch_fn_t(bool persistent)
+static const void* batch_fn_t(int v, bool persistent)
 {
-    return persistent ? reinterpret_cast<const void*>(&stft_batch_kernel<LOG_M>)
-                      : reinterpret_cast<const void*>(&stft_batch_dispatch_kernel<LOG_M>);
+    const void* fn = nullptr;
+    (void)with_variant<LOG_M>(v, [&](auto vc) {
+        constexpr int V = decltype(vc)::value;
+        using C = Cfg<LOG_M, V>;
+        if constexpr (C::SHAPE == 0) {
+            fn = persistent ? reinterpret_cast<const void*>(&stft_batch_kernel<LOG_M, C::PLAN>)
+                            : reinterpret_cast<const void*>(&stft_batch_dispatch_kernel<LOG_M, V>);
+        } else {
+            fn = persistent ? nullptr : reinterpret_cast<const void*>(&stft_batch_dispatch_kernel<LOG_M, V>);
+        }
+        return hipSuccess;
+    });
+    return fn;
 }
 
 bool fused_supported(int log_m) { return log_m >= 5 && log_m <= 14; }
 
+// f(integral_constant<int, LOG_M>) for a runtime log_m in 5 .. 14
+template <typename R, typename F>
+static R with_log_m(int log_m, R fail, F&& f)
+{
+    R r = fail;
+    host_for<5, 15>([&](auto lc) {
+        if (log_m == decltype(lc)::value)
+            r = f(lc);
+    });
+    return r;
+}
+
 int fused_group_frames(int log_m)
 {
-    switch (log_m) {
-    case 5: return Cfg<5>::G;
-    case 6: return Cfg<6>::G;
-    case 7: return Cfg<7>::G;
-    case 8: return Cfg<8>::G;
-    case 9: return Cfg<9>::G;
-    case 10: return Cfg<10>::G;
-    case 11: return Cfg<11>::G;
-    case 12: return Cfg<12>::G;
-    case 13: return Cfg<13>::G;
-    case 14: return Cfg<14>::G;
-    default: return 1;
-    }
+    return with_log_m(log_m, 1, [](auto lc) { return Cfg<decltype(lc)::value>::G; });
 }
 
-hipError_t launch_batch(int log_m, const BatchClassArgs& b, uint32_t grid, bool persistent, hipStream_t s)
+uint64_t fused_tile_frames(uint64_t G, uint64_t N, uint64_t hop)
 {
-    switch (log_m) {
-    case 5: return launch_batch_t<5>(b, grid, persistent, s);
-    case 6: return launch_batch_t<6>(b, grid, persistent, s);
-    case 7: return launch_batch_t<7>(b, grid, persistent, s);
-    case 8: return launch_batch_t<8>(b, grid, persistent, s);
-    case 9: return launch_batch_t<9>(b, grid, persistent, s);
-    case 10: return launch_batch_t<10>(b, grid, persistent, s);
-    case 11: return launch_batch_t<11>(b, grid, persistent, s);
-    case 12: return launch_batch_t<12>(b, grid, persistent, s);
-    case 13: return launch_batch_t<13>(b, grid, persistent, s);
-    case 14: return launch_batch_t<14>(b, grid, persistent, s);
-    default: return hipErrorInvalidValue;
-    }
+    return hop > N ? std::min<uint64_t>(G, (G - 1) * N / hop + 1) : G;
 }
 
-const void* batch_kernel_fn(int log_m, bool persistent)
+bool fused_variant_ok(int log_m, int variant)
 {
-    switch (log_m) {
-    case 5: return batch_fn_t<5>(persistent);
-    case 6: return batch_fn_t<6>(persistent);
-    case 7: return batch_fn_t<7>(persistent);
-    case 8: return batch_fn_t<8>(persistent);
-    case 9: return batch_fn_t<9>(persistent);
-    case 10: return batch_fn_t<10>(persistent);
-    case 11: return batch_fn_t<11>(persistent);
-    case 12: return batch_fn_t<12>(persistent);
-    case 13: return batch_fn_t<13>(persistent);
-    case 14: return batch_fn_t<14>(persistent);
-    default: return nullptr;
-    }
+    return with_log_m(log_m, false, [&](auto lc) {
+        bool ok = false;
+        host_for<0, FUSED_VARIANTS>([&](auto vc) {
+            if (variant == decltype(vc)::value)
+                ok = variant_ok<decltype(lc)::value, decltype(vc)::value>();
+        });
+        return ok;
+    });
 }
 
-hipError_t batch_class_info(int log_m, BatchClassInfo* info)
+bool fused_plan_ok(int log_m, int plan) { return plan == 0 || (plan == 1 && fused_variant_ok(log_m, 1)); }
+
+hipError_t launch_batch(int log_m, int variant, const BatchClassArgs& b, uint32_t grid, bool persistent,
+                        hipStream_t s)
 {
-    switch (log_m) {
-    case 5: return batch_info_t<5>(info);
-    case 6: return batch_info_t<6>(info);
-    case 7: return batch_info_t<7>(info);
-    case 8: return batch_info_t<8>(info);
-    case 9: return batch_info_t<9>(info);
-    case 10: return batch_info_t<10>(info);
-    case 11: return batch_info_t<11>(info);
-    case 12: return batch_info_t<12>(info);
-    case 13: return batch_info_t<13>(info);
-    case 14: return batch_info_t<14>(info);
-    default: return hipErrorInvalidValue;
-    }
+    return with_log_m(log_m, hipErrorInvalidValue, [&](auto lc) {
+        return launch_batch_t<decltype(lc)::value>(variant, b, grid, persistent, s);
+    });
+}
+
+const void* batch_kernel_fn(int log_m, int variant, bool persistent)
+{
+    return with_log_m(log_m, (const void*)nullptr,
+                      [&](auto lc) { return batch_fn_t<decltype(lc)::value>(variant, persistent); });
+}
+
+hipError_t batch_class_info(int log_m, int variant, BatchClassInfo* info)
+{
+    return with_log_m(log_m, hipErrorInvalidValue,
+                      [&](auto lc) { return batch_info_t<decltype(lc)::value>(variant, info); });
 }
 
 hipError_t launch_fused(int log_m, int xf, const FusedArgs& a, uint32_t count, hipStream_t s)
 {
-    switch (log_m) {
-    case 5: return launch_xf<5>(xf, a, count, s);
-    case 6: return launch_xf<6>(xf, a, count, s);
-    case 7: return launch_xf<7>(xf, a, count, s);
-    case 8: return launch_xf<8>(xf, a, count, s);
-    case 9: return launch_xf<9>(xf, a, count, s);
-    case 10: return launch_xf<10>(xf, a, count, s);
-    case 11: return launch_xf<11>(xf, a, count, s);
-    case 12: return launch_xf<12>(xf, a, count, s);
-    case 13: return launch_xf<13>(xf, a, count, s);
-    case 14: return launch_xf<14>(xf, a, count, s);
-    default: return hipErrorInvalidValue;
-    }
+    return with_log_m(log_m, hipErrorInvalidValue,
+                      [&](auto lc) { return launch_xf<decltype(lc)::value>(xf, a, count, s); });
 }
 
 }  // namespace ostft

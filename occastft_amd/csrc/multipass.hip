@@ -195,7 +195,7 @@ __global__ void __launch_bounds__(MP_THREADS) first_pass_kernel(const PassArgs a
     for (int j = 0; j < 16; ++j)
         v[j] = z[pad(t + j * T)];
     __syncthreads();
-    stockham<LOG_R, 0>(v, z, tws, a.tw_r, t);
+    stockham<LOG_R, 0>(v, z, tws, t);
 
     // Y[g R + q]: each butterfly's R outputs are contiguous
     const uint64_t obase = (P - a.frame0) * M + ((uint64_t)g0 << LOG_R);
@@ -291,7 +291,7 @@ __global__ void __launch_bounds__(MP_THREADS) last_pass_kernel(const LastArgs a)
     for (int j = 0; j < 16; ++j)
         v[j] = z[pad(t + j * T)];
     __syncthreads();
-    stockham<LOG_R, 0>(v, z, tws, a.tw_r, t);
+    stockham<LOG_R, 0>(v, z, tws, t);
     // column c, row q now holds Z[k], k = butterfly(c) + q Ns
 
     // split: Z[M - k] is column mirror(c), row R - 1 - q (row (R - q) mod R

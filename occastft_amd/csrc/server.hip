@@ -451,8 +451,8 @@ int ostft_server_create(int32_t device, const ostft_server_config* config, ostft
     if (e == hipSuccess)
         rc = ostft_batch_config(s->ctx, 0, 0, cfg.use_graphs ? 1 : 0, nullptr);
     if (e == hipSuccess && rc == OSTFT_OK && cfg.autotune) {
-        for (int exp = 6; exp <= 14 && rc == OSTFT_OK; ++exp)
-            rc = ostft_autotune_batch(s->ctx, exp, 1ull << 26, nullptr, nullptr);
+        for (int exp = 6; exp <= 15 && rc == OSTFT_OK; ++exp)
+            rc = ostft_autotune_batch(s->ctx, exp, 1ull << 26, nullptr, nullptr, nullptr, nullptr);
     }
     if (e != hipSuccess || rc != OSTFT_OK) {
         if (s->arena_d)

@@ -29,18 +29,30 @@
 
 namespace ostft {
 
-template <int LOG_M>
+// Shape and radix plan of the engine for complex size M = 2^LOG_M.  Variant
+// V: bit 0 is the radix plan (0: the radix-16 passes, then the remainder radix
+// 2/4/8; 1: the remainder radix first), bits 1-2 the workgroup (0: at least
+// OSTFT_WG_MIN threads, 1: at least 512, 2: at least 128).  A transform's
+// arithmetic depends on the plan only; the workgroup decides how many
+// transforms share one.
+template <int LOG_M, int V = 0>
 struct Cfg {
     static constexpr int M = 1 << LOG_M;
     static constexpr int T = M / 16;               // threads per transform
-    static constexpr int WG_MIN = LOG_M >= OSTFT_BIG_LOG_M ? OSTFT_WG_BIG : OSTFT_WG_MIN;
+    static constexpr int PLAN = V & 1, SHAPE = V >> 1;
+    static constexpr int WG_MIN = SHAPE == 1   ? 512
+                                  : SHAPE == 2 ? 128
+                                               : (LOG_M >= OSTFT_BIG_LOG_M ? OSTFT_WG_BIG : OSTFT_WG_MIN);
     static constexpr int WG = T < WG_MIN ? WG_MIN : T;  // threads per workgroup
     static constexpr int G = WG / T;               // transforms per workgroup
     static constexpr int PADM = M + M / 16;        // padded LDS length per transform
     static constexpr int NP16 = LOG_M / 4;
     static constexpr int REM = LOG_M % 4;
     static constexpr int NPASS = NP16 + (REM ? 1 : 0);
-    static constexpr int radix(int i) { return i < NP16 ? 16 : (1 << REM); }
+    static constexpr int radix(int i)
+    {
+        return (PLAN && REM) ? (i == 0 ? (1 << REM) : 16) : (i < NP16 ? 16 : (1 << REM));
+    }
     static constexpr int ns(int i)
     {
         int n = 1;
@@ -122,87 +134,69 @@ __device__ __forceinline__ float2 group_sum(float2 s, float2* red, uint32_t tid)
 
 // Twiddles of every pass for one thread.  Pass p (radix r, stride NS > 1)
 // multiplies input m of butterfly g = t + u*T by w^m with
-// w = W_M^{(g mod NS) * M/(NS r)}.  The values depend only on the thread's
-// index, so they are loaded once, up front, with the tile's other global
-// loads.  OSTFT_TW_ALL = 0 (default): the thread loads w (and w^4, w^8 for
-// radix 8/16) and forms the other powers by products; 1: every power w^m
-// (m < r) comes from the table, r - 1 loads per butterfly and no products
-// (measured slower from window 1024 on: 0.32 against 0.24 ms at 4096 with 50%
-// overlap, although it runs fewer vector ALU instructions).
-#ifndef OSTFT_TW_ALL
-#define OSTFT_TW_ALL 0
-#endif
-template <int LOG_M>
+// w = W_M^{(g mod NS) * M/(NS r)}; the thread keeps w (and w^4, w^8 for
+// radix 8/16) and forms the other powers by products.  The values depend only
+// on the thread's index, so they are loaded once, up front, with the tile's
+// other global loads.  (Every power from the table instead -- more loads,
+// fewer vector instructions -- measured slower from window 1024 on: 0.32
+// against 0.24 ms at 4096 with 50% overlap.)
+template <int LOG_M, int V = 0>
 struct PassTwiddles {
-#if OSTFT_TW_ALL
-    float2 b[Cfg<LOG_M>::NPASS][16];  // [pass][u r + m] = w_u^m, m = 1 .. r-1; unused slots vanish
-#else
-    float2 b[Cfg<LOG_M>::NPASS][16][3];  // [pass][butterfly u][w, w^4, w^8]; unused slots vanish
-#endif
+    float2 b[Cfg<LOG_M, V>::NPASS][16][3];  // [pass][butterfly u][w, w^4, w^8]; unused slots vanish
 };
 
-// the twiddles of pass PP (NS > 1) for this thread
-template <int LOG_M, int PP>
-__device__ __forceinline__ void load_pass_tw(PassTwiddles<LOG_M>& P, const float2* __restrict__ tw, uint32_t t)
+template <int LOG_M, int V = 0>
+__device__ __forceinline__ void load_pass_twiddles(PassTwiddles<LOG_M, V>& P, const float2* __restrict__ tw,
+                                                   uint32_t t)
 {
-    using C = Cfg<LOG_M>;
+    using C = Cfg<LOG_M, V>;
     constexpr int M = C::M, T = C::T;
-    constexpr int r = C::radix(PP), NS = C::ns(PP), U = 16 / r, STEP = M / (NS * r);
-    cfor<0, U>([&](auto u) {
-        constexpr int UU = decltype(u)::value;
-        const uint32_t e1 = ((t + UU * T) & (NS - 1)) * STEP;
-#if OSTFT_TW_ALL
-        // m e1 < r NS STEP = M: inside the table
-        cfor<1, r>([&](auto m) {
-            constexpr int MM = decltype(m)::value;
-            P.b[PP][UU * r + MM] = tw[MM * e1];
+    cfor<1, C::NPASS>([&](auto p) {
+        constexpr int PP = decltype(p)::value;
+        constexpr int r = C::radix(PP), NS = C::ns(PP), U = 16 / r, STEP = M / (NS * r);
+        cfor<0, U>([&](auto u) {
+            constexpr int UU = decltype(u)::value;
+            const uint32_t e1 = ((t + UU * T) & (NS - 1)) * STEP;
+            P.b[PP][UU][0] = tw[e1];
+            if constexpr (r >= 8)
+                P.b[PP][UU][1] = tw[4 * e1];
+            if constexpr (r == 16)
+                P.b[PP][UU][2] = tw[8 * e1];
         });
-#else
-        P.b[PP][UU][0] = tw[e1];
-        if constexpr (r >= 8)
-            P.b[PP][UU][1] = tw[4 * e1];
-        if constexpr (r == 16)
-            P.b[PP][UU][2] = tw[8 * e1];
-#endif
     });
 }
 
-// The twiddles loaded with the tile's other global loads: every pass's in
-// the products form; none in the table form, where each pass's are loaded one
-// pass ahead, in stockham (with the samples and the window, or all at once,
-// they would not fit the registers).
-template <int LOG_M>
-__device__ __forceinline__ void load_pass_twiddles(PassTwiddles<LOG_M>& P, const float2* __restrict__ tw,
-                                                   uint32_t t)
+// The first pass expects, for butterfly u of the thread, v[u r + m] =
+// X[t + u T + m M/r]; the gather delivers v[j] = X[t + j T].  For a first
+// radix below 16 (plan 1) that is a fixed renaming of the 16 registers.
+template <int LOG_M, int V = 0>
+__device__ __forceinline__ void first_pass_order(float2 (&v)[16])
 {
-#if OSTFT_TW_ALL
-    (void)P, (void)tw, (void)t;
-#else
-    cfor<1, Cfg<LOG_M>::NPASS>([&](auto p) { load_pass_tw<LOG_M, decltype(p)::value>(P, tw, t); });
-#endif
+    using C = Cfg<LOG_M, V>;
+    constexpr int r = C::radix(0), U = 16 / r;
+    if constexpr (r != 16) {
+        float2 w[16];
+        cfor<0, U>([&](auto u) {
+            constexpr int UU = decltype(u)::value;
+            cfor<0, r>([&](auto m) {
+                constexpr int MM = decltype(m)::value;
+                w[UU * r + MM] = v[UU + MM * U];  // X[t + (u + m U) T], m U T = m M/r
+            });
+        });
+        cfor<0, 16>([&](auto j) { v[decltype(j)::value] = w[decltype(j)::value]; });
+    }
 }
 
 // Stockham pass PASS: twiddle, radix-r DFTs, scatter to LDS; then gather the
-// inputs of the next pass (or stop with the spectrum in LDS).  The only
-// global memory touched is the twiddle table (the table form's loads one pass
-// ahead).
-template <int LOG_M, int PASS>
-__device__ __forceinline__ void stockham(float2 (&v)[16], float2* z, PassTwiddles<LOG_M>& P,
-                                         const float2* __restrict__ tw, uint32_t t)
+// inputs of the next pass (or stop with the spectrum in LDS).  No global
+// memory is touched here.
+template <int LOG_M, int PASS, int V = 0>
+__device__ __forceinline__ void stockham(float2 (&v)[16], float2* z, const PassTwiddles<LOG_M, V>& P, uint32_t t)
 {
-    using C = Cfg<LOG_M>;
+    using C = Cfg<LOG_M, V>;
     constexpr int M = C::M, T = C::T;
     constexpr int r = C::radix(PASS), NS = C::ns(PASS), U = 16 / r;
     if constexpr (NS > 1) {
-#if OSTFT_TW_ALL
-        cfor<0, U>([&](auto u) {
-            constexpr int UU = decltype(u)::value;
-            cfor<1, r>([&](auto m) {
-                constexpr int MM = decltype(m)::value;
-                v[UU * r + MM] = cmul(v[UU * r + MM], P.b[PASS][UU * r + MM]);
-            });
-        });
-#else
         cfor<0, U>([&](auto u) {
             constexpr int UU = decltype(u)::value;
             float2 w[16];
@@ -229,18 +223,7 @@ __device__ __forceinline__ void stockham(float2 (&v)[16], float2* z, PassTwiddle
                 v[UU * r + MM] = cmul(v[UU * r + MM], w[MM]);
             });
         });
-#endif
     }
-#if OSTFT_TW_ALL
-    if constexpr (PASS + 1 < C::NPASS) {
-        // the next pass's twiddles, issued once this pass's are applied (not
-        // hoisted above: both sets together would not fit the registers)
-        asm volatile("" ::: "memory");
-        load_pass_tw<LOG_M, PASS + 1>(P, tw, t);
-    }
-#else
-    (void)tw;
-#endif
     cfor<0, U>([&](auto u) {
         constexpr int UU = decltype(u)::value;
         dft<r, UU * r>(v);
@@ -266,7 +249,7 @@ __device__ __forceinline__ void stockham(float2 (&v)[16], float2* z, PassTwiddle
             });
         });
         __syncthreads();
-        stockham<LOG_M, PASS + 1>(v, z, P, tw, t);
+        stockham<LOG_M, PASS + 1, V>(v, z, P, t);
     }
 }
 

@@ -16,7 +16,7 @@ ActivateSTFT call).
 from __future__ import annotations
 
 import ctypes
-from typing import Optional
+from typing import NamedTuple, Optional
 
 import numpy as np
 
@@ -210,15 +210,43 @@ class Runner:
             raise STFTError(rc, self.last_error)
         return cur.value
 
-    def autotune_batch(self, exp: int, samples: int = 1 << 24):
-        """-> (best setting, best ms) for window exponent `exp`: 0 = one
-        workgroup per tile, k > 0 = persistent with k workgroups per CU."""
-        best = ctypes.c_int32()
-        ms = ctypes.c_double()
-        rc = N.lib().ostft_autotune_batch(self._h, int(exp), int(samples), ctypes.byref(best), ctypes.byref(ms))
+    def engine_variant(self, exp: int, shape: int = -1, plan: int = -1):
+        """-> (shape, plan) in force for window exponent `exp` (6..15), after
+        setting either (-1 leaves it).  plan: radix plan of the FP32 engine
+        for every request of that window on this context (0: radix-16 passes
+        first, 1: the remainder radix first; exponents 9 and 13 have plan 0
+        only).  shape: frames per workgroup of the batch kernels (0: 256
+        threads, 1: 512, 2: 128; shapes 1 and 2 run one workgroup per tile)."""
+        cs = ctypes.c_int32()
+        cp = ctypes.c_int32()
+        rc = N.lib().ostft_engine_variant(self._h, int(exp), int(shape), int(plan), ctypes.byref(cs),
+                                          ctypes.byref(cp))
         if rc != N.OK:
             raise STFTError(rc, self.last_error)
-        return best.value, ms.value
+        return cs.value, cp.value
+
+    def autotune_batch(self, exp: int, samples: int = 1 << 24) -> "AutotuneChoice":
+        """Times every built variant and launch form of window exponent `exp`
+        on a synthetic ragged batch, keeps the fastest for this context and
+        returns it: blocks_per_cu (0 = one workgroup per tile, k > 0 =
+        persistent with k workgroups per CU), shape, plan, ms."""
+        best = ctypes.c_int32()
+        shape = ctypes.c_int32()
+        plan = ctypes.c_int32()
+        ms = ctypes.c_double()
+        rc = N.lib().ostft_autotune_batch(self._h, int(exp), int(samples), ctypes.byref(best), ctypes.byref(shape),
+                                          ctypes.byref(plan), ctypes.byref(ms))
+        if rc != N.OK:
+            raise STFTError(rc, self.last_error)
+        return AutotuneChoice(best.value, shape.value, plan.value, ms.value)
+
+
+class AutotuneChoice(NamedTuple):
+    """What Runner.autotune_batch kept for one window size."""
+    blocks_per_cu: int  # 0: one workgroup per tile; k > 0: persistent, k per CU
+    shape: int          # workgroup shape of the batch kernels
+    plan: int           # radix plan of the FP32 engine
+    ms: float           # GPU time of one synthetic batch
 
 
 class Server:

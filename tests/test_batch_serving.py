@@ -179,9 +179,10 @@ def test_empty_batch(runner):
 def test_autotune_keeps_results(runner):
     import torch
 
-    best, ms = runner.autotune_batch(10, 1 << 24)
-    assert best >= 0 and ms > 0  # 0: one workgroup per tile
-    assert runner.batch_config(10) == best
+    choice = runner.autotune_batch(10, 1 << 24)
+    assert choice.blocks_per_cu >= 0 and choice.ms > 0  # 0: one workgroup per tile
+    assert runner.batch_config(10) == choice.blocks_per_cu
+    assert runner.engine_variant(10) == (choice.shape, choice.plan)
     reqs = ragged_requests(31, 12, exps=[10])
     alone = [run_alone(runner, x, p) for x, p in reqs]
     xs = device_inputs(reqs)
@@ -195,6 +196,7 @@ def test_autotune_keeps_results(runner):
         torch.cuda.synchronize()
         for o, a in zip(outs, alone):
             assert bitwise_equal(o.cpu().numpy(), a)
+    runner.engine_variant(10, 0, 0)
 
 
 # ------------------------------------------------------------------ serving

@@ -32,7 +32,7 @@ def test_library_exports_every_declared_symbol():
     assert syms == set(N.EXPORTS)
     for s in syms:
         assert hasattr(L, s), s
-    assert L.ostft_abi_version() == 1
+    assert L.ostft_abi_version() == N.ABI_VERSION == 2
 
 
 def test_native_missing_is_loud(monkeypatch, tmp_path):
