@@ -58,6 +58,38 @@ struct BatchWaves {
     static constexpr int value = (LOG_M == 9 || LOG_M == 11 || LOG_M >= 13) ? 1 : 4;
 };
 
+// The per-frame slot epilogue of the N/2+1 layouts is used from
+// OSTFT_SLOT_MIN_T threads per transform (windows of 32 x that) on.
+#ifndef OSTFT_SLOT_MIN_T
+#define OSTFT_SLOT_MIN_T 16
+#endif
+// Stores of the N/2+1 layouts (POWER, HALF_SPECTRUM): plain by default.  A
+// frame's M+1 values start at any 4- or 8-byte boundary, so a wave's store
+// covers partial cache lines at its ends; plain stores let L2 merge them
+// with the neighbouring waves' parts, where streaming stores (OSTFT_HALF_NT=1)
+// measured up to 1.9x slower (window 512, 2^27 samples).  The N-value
+// layouts store whole 16-byte blocks per lane and keep streaming stores.
+#ifndef OSTFT_HALF_NT
+#define OSTFT_HALF_NT 0
+#endif
+template <typename V>
+__device__ __forceinline__ void half_store(V v, V* p)
+{
+#if OSTFT_HALF_NT
+    nt_store(v, p);
+#else
+    *p = v;
+#endif
+}
+__device__ __forceinline__ void half_store4(float4 v, float4* p)
+{
+#if OSTFT_HALF_NT
+    nt_store4(v, p);
+#else
+    *p = v;
+#endif
+}
+
 // FusedArgs as the batch kernel reads it from the descriptor block: the same
 // layout, with the pointers typed as global memory.  A generic pointer read
 // from memory would compile to flat accesses; one cast from a global pointer
@@ -227,6 +259,15 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
     // and iteration; at the smallest window two consecutive values of the
     // flattened [frame][bin] index)
     constexpr int OITERS = (G * (M / 2 + 1) + WG - 1) / WG;
+    // from T >= 16 (windows from 512) each group writes its own frame's M/2+1
+    // slots, k = t + j T (j < 8) and k = M/2 from t == 0: no per-slot index
+    // division, and every address is the thread's base plus a constant
+    constexpr bool SLOT_FRAME = SYM_REAL && T >= OSTFT_SLOT_MIN_T;
+    // at the largest transforms (one frame per workgroup, four passes) the
+    // slot twiddles are loaded after the FFT, which keeps 18 registers free
+    // through it
+    constexpr bool SLOT_LATE_TW = SLOT_FRAME && T >= 512;
+    static_assert(!SLOT_FRAME || OITERS >= 9, "slot twiddles");
     constexpr int OITERP = (G * (M + 1) + 2 * WG - 1) / (2 * WG);
     constexpr int OITERM = OITERS > OITERP ? OITERS : OITERP;
     constexpr int NEB = OITERM > PITER ? (OITERM > OITER4 ? OITERM : OITER4) : (PITER > OITER4 ? PITER : OITER4);
@@ -258,7 +299,14 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
         ec3 = twn[3];
         ec4 = twn[4];
     } else if (a.combine) {
-        if constexpr (SYM_REAL) {
+        if constexpr (SLOT_FRAME) {
+            if constexpr (!SLOT_LATE_TW) {
+#pragma unroll
+                for (int j = 0; j < 8; ++j)
+                    eb[j] = twn[t + j * T];
+                eb[8] = twn[M / 2];
+            }
+        } else if constexpr (SYM_REAL) {
 #pragma unroll
             for (int it = 0; it < OITERS; ++it) {
                 const uint32_t e = tid + it * WG;
@@ -490,6 +538,53 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
                     nt_store4(make_float4(hi[0], hi[1], hi[2], hi[3]), o + (M + k0) / 4);
                 }
             }
+        } else if constexpr (SLOT_FRAME) {
+            // POWER / HALF_SPECTRUM, M+1 values per frame, one frame per group
+            // (slots as in the branch below).  With T a multiple of 16,
+            // pad(t + jT) = pad(t) + 17 jT/16 and pad(M - jT - t) =
+            // 17 (M - jT)/16 - (t + ceil(t/16)): every LDS and output address
+            // is a per-thread base plus a constant.  (Slot 0's partner M - 0
+            // wraps to 0.)
+            if constexpr (SLOT_LATE_TW) {
+                if (COMB) {
+#pragma unroll
+                    for (int j = 0; j < 8; ++j)
+                        eb[j] = twn[t + j * T];
+                    eb[8] = twn[M / 2];
+                }
+            }
+            if (active) {
+                constexpr uint32_t PT = T + T / 16;  // padded stride of j
+                const float2* z = lds + grp * PADM;
+                const float2* zl = z + pad(t);
+                const float2* zh = z + (PADM - 7 * PT) - (t + ((t + 15u) >> 4));  // + (7 - j) PT
+                const uint64_t o = (f0 + grp) * (uint64_t)(M + 1);
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    const uint32_t k = t + j * T;
+                    const float2 zm = (j == 0 && t == 0) ? z[0] : zh[(7 - j) * PT];
+                    const BinPair b = finish_pair<COMB>(zl[j * PT], zm, eb[j]);
+                    const float2 x2 = (COMB || k == 0) ? b.hi : b.lo;  // position M-k, before conj
+                    if constexpr (MODE == OUT_POWER) {
+                        float* ol = out + o + t;
+                        float* oh = out + o + (M - t);
+                        half_store(power_of(b.lo), ol + j * T);
+                        half_store(power_of(x2), oh - j * T);
+                    } else {
+                        float2* ol = reinterpret_cast<float2*>(out) + o + t;
+                        float2* oh = reinterpret_cast<float2*>(out) + o + (M - t);
+                        half_store(value_of(b.lo), ol + j * T);
+                        half_store(k == 0 ? value_of(x2) : conj_value_of(x2), oh - j * T);
+                    }
+                }
+                if (t == 0) {  // slot M/2: its own mirror
+                    const BinPair b = finish_pair<COMB>(z[pad(M / 2)], z[pad(M / 2)], eb[8]);
+                    if constexpr (MODE == OUT_POWER)
+                        half_store(power_of(b.lo), out + o + M / 2);
+                    else
+                        half_store(value_of(b.lo), reinterpret_cast<float2*>(out) + o + M / 2);
+                }
+            }
         } else if constexpr (SYM_REAL) {
             // POWER / HALF_SPECTRUM: M+1 values per frame.  Slot k = 0 .. M/2 of
             // a frame gives positions k and M-k from one pair (Z[k], Z[M-k]):
@@ -514,14 +609,14 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
                 const bool second = k != (uint32_t)(M / 2);
                 const uint64_t o = obase + (uint64_t)g * (M + 1);
                 if constexpr (MODE == OUT_POWER) {
-                    nt_store(power_of(x1), out + o + k);
+                    half_store(power_of(x1), out + o + k);
                     if (second)
-                        nt_store(power_of(x2), out + o + (M - k));
+                        half_store(power_of(x2), out + o + (M - k));
                 } else {
                     float2* o2 = reinterpret_cast<float2*>(out) + o;
-                    nt_store(value_of(x1), o2 + k);
+                    half_store(value_of(x1), o2 + k);
                     if (second)
-                        nt_store(k == 0 ? value_of(x2) : conj_value_of(x2), o2 + (M - k));
+                        half_store(k == 0 ? value_of(x2) : conj_value_of(x2), o2 + (M - k));
                 }
             }
         } else {
@@ -560,21 +655,21 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
                 if constexpr (MODE == OUT_POWER) {
                     const float p1 = power_of(x1), p2 = power_of(x2);
                     if (pair) {
-                        nt_store(make_float2(p1, p2), reinterpret_cast<float2*>(out + o));
+                        half_store(make_float2(p1, p2), reinterpret_cast<float2*>(out + o));
                     } else {
-                        nt_store(p1, out + o);
+                        half_store(p1, out + o);
                         if (has2)
-                            nt_store(p2, out + o + 1);
+                            half_store(p2, out + o + 1);
                     }
                 } else {
                     float2* o2 = reinterpret_cast<float2*>(out) + o;
                     const float2 v1 = value_of(x1), v2 = value_of(x2);
                     if (pair) {
-                        nt_store4(make_float4(v1.x, v1.y, v2.x, v2.y), reinterpret_cast<float4*>(o2));
+                        half_store4(make_float4(v1.x, v1.y, v2.x, v2.y), reinterpret_cast<float4*>(o2));
                     } else {
-                        nt_store(v1, o2);
+                        half_store(v1, o2);
                         if (has2)
-                            nt_store(v2, o2 + 1);
+                            half_store(v2, o2 + 1);
                     }
                 }
             }
