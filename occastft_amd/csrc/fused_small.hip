@@ -259,10 +259,14 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
     // and iteration; at the smallest window two consecutive values of the
     // flattened [frame][bin] index)
     constexpr int OITERS = (G * (M / 2 + 1) + WG - 1) / WG;
-    // from T >= 16 (windows from 512) each group writes its own frame's M/2+1
-    // slots, k = t + j T (j < 8) and k = M/2 from t == 0: no per-slot index
-    // division, and every address is the thread's base plus a constant
+    // From T >= 16 (windows from 512) the N/2+1 layouts are written frame by
+    // frame: slots k < M/2 of a frame as k = t + j T (j < 8) by its own group
+    // when a group spans whole waves (T >= 64), else as k = lane + 64 jj
+    // (jj < T/8) by a whole wave, frame after frame; the middle slot k = M/2
+    // apart.  No per-slot index division, every address a per-thread base
+    // plus a constant, and each store instruction one contiguous run.
     constexpr bool SLOT_FRAME = SYM_REAL && T >= OSTFT_SLOT_MIN_T;
+    constexpr bool SLOT_WAVE = SLOT_FRAME && T < 64;
     // at the largest transforms (one frame per workgroup, four passes) the
     // slot twiddles are loaded after the FFT, which keeps 18 registers free
     // through it
@@ -299,7 +303,13 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
         ec3 = twn[3];
         ec4 = twn[4];
     } else if (a.combine) {
-        if constexpr (SLOT_FRAME) {
+        if constexpr (SLOT_WAVE) {
+            // the slot of lane l and step jj is the same in every frame
+#pragma unroll
+            for (int jj = 0; jj < T / 8; ++jj)
+                eb[jj] = twn[(tid & 63u) + 64 * jj];
+            eb[T / 8] = twn[M / 2];
+        } else if constexpr (SLOT_FRAME) {
             if constexpr (!SLOT_LATE_TW) {
 #pragma unroll
                 for (int j = 0; j < 8; ++j)
@@ -538,9 +548,53 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
                     nt_store4(make_float4(hi[0], hi[1], hi[2], hi[3]), o + (M + k0) / 4);
                 }
             }
+        } else if constexpr (SLOT_WAVE) {
+            // POWER / HALF_SPECTRUM, M+1 values per frame, 64 / T frames per
+            // wave, one after the other (slots as in the SYM_REAL branch
+            // below).  pad(l + 64 jj) = pad(l) + 68 jj and pad(M - 64 jj - l)
+            // = PADM - 68 jj - (l + ceil(l/16)).
+            constexpr uint32_t FW = 64 / T, JW = T / 8;
+            const uint32_t lane = tid & 63u, wv = tid >> 6;
+            const uint32_t lu = lane + ((lane + 15u) >> 4);
+#pragma unroll
+            for (uint32_t fi = 0; fi < FW; ++fi) {
+                const uint32_t g = wv * FW + fi;
+                if (g < nf) {
+                    const float2* z = lds + g * PADM;
+                    const float2* zl = z + pad(lane);
+                    const float2* zh = z + (PADM - 68 * (JW - 1)) - lu;  // + 68 (JW - 1 - jj)
+                    const uint64_t o = (f0 + g) * (uint64_t)(M + 1);
+#pragma unroll
+                    for (uint32_t jj = 0; jj < JW; ++jj) {
+                        const uint32_t k = lane + 64 * jj;
+                        const float2 zm = (jj == 0 && lane == 0) ? z[0] : zh[68 * (JW - 1 - jj)];
+                        const BinPair b = finish_pair<COMB>(zl[68 * jj], zm, eb[jj]);
+                        const float2 x2 = (COMB || k == 0) ? b.hi : b.lo;  // position M-k, before conj
+                        if constexpr (MODE == OUT_POWER) {
+                            half_store(power_of(b.lo), out + o + lane + 64 * jj);
+                            half_store(power_of(x2), out + o + (M - lane) - 64 * jj);
+                        } else {
+                            float2* o2 = reinterpret_cast<float2*>(out) + o;
+                            half_store(value_of(b.lo), o2 + lane + 64 * jj);
+                            half_store(k == 0 ? value_of(x2) : conj_value_of(x2), o2 + (M - lane) - 64 * jj);
+                        }
+                    }
+                }
+            }
+            // the middle slots of the wave's frames, one per lane
+            const uint32_t gm = wv * FW + lane;
+            if (lane < FW && gm < nf) {
+                const float2* z = lds + gm * PADM;
+                const BinPair b = finish_pair<COMB>(z[pad(M / 2)], z[pad(M / 2)], eb[JW]);
+                const uint64_t o = (f0 + gm) * (uint64_t)(M + 1) + M / 2;
+                if constexpr (MODE == OUT_POWER)
+                    half_store(power_of(b.lo), out + o);
+                else
+                    half_store(value_of(b.lo), reinterpret_cast<float2*>(out) + o);
+            }
         } else if constexpr (SLOT_FRAME) {
             // POWER / HALF_SPECTRUM, M+1 values per frame, one frame per group
-            // (slots as in the branch below).  With T a multiple of 16,
+            // (slots as in the SYM_REAL branch below).  With T a multiple of 16,
             // pad(t + jT) = pad(t) + 17 jT/16 and pad(M - jT - t) =
             // 17 (M - jT)/16 - (t + ceil(t/16)): every LDS and output address
             // is a per-thread base plus a constant.  (Slot 0's partner M - 0
