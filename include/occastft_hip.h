@@ -222,17 +222,20 @@ int ostft_batch_config(ostft_ctx* ctx, int32_t exp, int32_t blocks_per_cu, int32
                        int32_t* current);
 
 /* Engine variant of window exponent `exp` (6..15) on this context.
- *   plan   radix plan of the FP32 single-kernel transform: 0 runs the radix-16
- *          passes first and the remainder radix (2, 4 or 8) last, 1 the
- *          remainder first.  Exponents 9 and 13 have no remainder (plan 0
- *          only).  Every FP32 request of that window on the context uses it
- *          (ostft_run_device, _noalloc, _strided, batches), so batch results
- *          stay bitwise equal to single requests; the two plans agree to
- *          rounding, not bitwise.
+ *   plan   radix plan of the FP32 single-kernel transform: 0 radix-16 passes
+ *          (16 values per thread) with the remainder radix (2, 4 or 8) last,
+ *          1 the remainder first; 2 and 3 the same with radix-8 passes (8
+ *          values per thread, twice the threads per transform; exponents
+ *          8..14).  Plans 1 and 3 need a remainder: plan 1 is not built for
+ *          exponents 9 and 13, plan 3 not for 10 and 13.  Every FP32 request
+ *          of that window on the context uses the plan (ostft_run_device,
+ *          _noalloc, _strided, batches), so batch results stay bitwise equal
+ *          to single requests; plans agree to rounding, not bitwise.
  *   shape  frames per workgroup of the batch kernels: 0 the default (256
  *          threads, or one transform when it needs more), 1 512 threads
- *          (exponents up to 13), 2 128 threads (up to 12).  Shapes 1 and 2
- *          run in the one-workgroup-per-tile form.
+ *          (exponents up to 13), 2 128 threads (up to 12); plans 2 and 3 have
+ *          shape 0 only.  Shapes 1 and 2 run in the one-workgroup-per-tile
+ *          form.
  * -1 leaves a setting; *cur_shape / *cur_plan (may be NULL) receive what is
  * in force.  A plan or shape not built for `exp` is OSTFT_ERR_UNSUPPORTED.
  * No counterpart in the reference, whose radix plan is fixed per size. */

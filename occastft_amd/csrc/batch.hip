@@ -56,7 +56,7 @@ struct BatchState {
     int use_graphs = 1;
     int persistent[NCLS] = {};  // 1: persistent grid claiming tiles; 0: one workgroup per tile
     int per_cu[NCLS] = {};      // persistent workgroups per CU; 0: the occupancy limit
-    int shape[NCLS] = {};       // workgroup shape (engine variant bits 1-2); persistent needs 0
+    int shape[NCLS] = {};       // workgroup shape (engine variant / 4); persistent needs 0
     int plan_override[NCLS];    // >= 0: radix plan for this class instead of the context's (autotuning)
     int variant[NCLS] = {};     // engine variant of each class in the current batch
     BatchClassInfo info[NCLS][FUSED_VARIANTS] = {};
@@ -135,8 +135,13 @@ int class_info(ostft_ctx* ctx, BatchState* st, int c, int v)
     return OSTFT_OK;
 }
 
-// Engine variant of class c: its workgroup shape with radix plan `plan`.
-int class_variant(const BatchState* st, int c, int plan) { return st->shape[c] * 2 + plan; }
+// Engine variant of class c: its workgroup shape with radix plan `plan`, or
+// the default shape where that pair is not built.
+int class_variant(const BatchState* st, int c, int plan)
+{
+    const int v = st->shape[c] * 4 + plan;
+    return fused_variant_ok(c + LOG_M_MIN, v) ? v : plan;
+}
 
 // Grid of class c for a batch with `ntiles` tiles of it.
 uint32_t class_grid(const BatchState* st, int c, uint64_t ntiles)
@@ -496,16 +501,17 @@ int engine_variant(ostft_ctx* ctx, int exp, int shape, int plan, int* cur_shape,
     if (plan >= 0 && !fused_plan_ok(log_m, plan))
         return ctx_fail(ctx, OSTFT_ERR_UNSUPPORTED, "radix plan " + std::to_string(plan) +
                                                         " is not built for window exponent " + std::to_string(exp));
-    if (shape >= 0 && (shape > (FUSED_VARIANTS - 1) / 2 || !fused_variant_ok(log_m, shape * 2)))
-        return ctx_fail(ctx, OSTFT_ERR_UNSUPPORTED, "workgroup shape " + std::to_string(shape) +
-                                                        " is not built for window exponent " + std::to_string(exp));
+    const int new_shape = shape >= 0 ? shape : st->shape[c];
+    const int new_plan = plan >= 0 ? plan : ctx_radix_plan(ctx, log_m);
+    if (new_shape < 0 || new_shape > 2 || !fused_variant_ok(log_m, new_shape * 4 + new_plan))
+        return ctx_fail(ctx, OSTFT_ERR_UNSUPPORTED,
+                        "workgroup shape " + std::to_string(new_shape) + " with radix plan " + std::to_string(new_plan) +
+                            " is not built for window exponent " + std::to_string(exp));
     if (plan >= 0)
         ctx_set_radix_plan(ctx, log_m, plan);
-    if (shape >= 0) {
-        st->shape[c] = shape;
-        if (shape != 0)
-            st->persistent[c] = 0;  // the persistent kernel has shape 0 only
-    }
+    st->shape[c] = new_shape;
+    if (new_shape != 0)
+        st->persistent[c] = 0;  // the persistent kernel has shape 0 only
     if (cur_shape)
         *cur_shape = st->shape[c];
     if (cur_plan)
@@ -542,7 +548,7 @@ int autotune_batch(ostft_ctx* ctx, int exp, uint64_t samples, int* best_blocks_p
         if (rc != OSTFT_OK)
             return rc;
         cand.push_back({v, 0});
-        if (v < 2) {  // shape 0: the persistent form too
+        if (v < 4) {  // shape 0: the persistent form too
             const int mx = st->info[c][v].max_blocks_per_cu;
             for (int k = 1; k < mx; k *= 2)
                 cand.push_back({v, k});
@@ -609,8 +615,8 @@ int autotune_batch(ostft_ctx* ctx, int exp, uint64_t samples, int* best_blocks_p
         std::pair<int, int> best{0, 0};
         float best_t = 1e30f;
         for (const auto& [v, k] : cand) {
-            st->shape[c] = v >> 1;
-            st->plan_override[c] = v & 1;
+            st->shape[c] = v >> 2;
+            st->plan_override[c] = v & 3;
             st->persistent[c] = k > 0;
             st->per_cu[c] = k;
             // run; e0; run; e1: the second batch is prepared while the first
@@ -647,16 +653,16 @@ int autotune_batch(ostft_ctx* ctx, int exp, uint64_t samples, int* best_blocks_p
         st->shape[c] = saved_shape;
         if (rc == OSTFT_OK && e == hipSuccess) {
             // captured launches take it at their next replay
-            st->shape[c] = best.first >> 1;
+            st->shape[c] = best.first >> 2;
             st->persistent[c] = best.second > 0;
             st->per_cu[c] = best.second;
-            ctx_set_radix_plan(ctx, log_m, best.first & 1);
+            ctx_set_radix_plan(ctx, log_m, best.first & 3);
             if (best_blocks_per_cu)
                 *best_blocks_per_cu = best.second;
             if (best_shape)
-                *best_shape = best.first >> 1;
+                *best_shape = best.first >> 2;
             if (best_plan)
-                *best_plan = best.first & 1;
+                *best_plan = best.first & 3;
             if (best_ms)
                 *best_ms = best_t;
         }

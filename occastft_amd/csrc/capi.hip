@@ -416,9 +416,11 @@ int ctx_prepare(ostft_ctx* ctx, const ostft_params* params, const float* d_in, u
         // a tile of G frames stages G N samples; a hop beyond the window (only
         // reachable with an explicit geometry) takes fewer frames per tile, so
         // that (frames - 1) hop + N <= G N
-        a->tile_frames = (int32_t)fused_tile_frames((uint64_t)fused_group_frames(plan.log_m), (uint64_t)N,
-                                                    plan.g.hop);
         a->plan = ctx_radix_plan(ctx, plan.log_m);
+        if (!fused_plan_ok(plan.log_m, a->plan))
+            a->plan = 0;
+        a->tile_frames = (int32_t)fused_tile_frames((uint64_t)fused_variant_frames(plan.log_m, a->plan),
+                                                    (uint64_t)N, plan.g.hop);
     }
     a->tw = nullptr;
     a->tw_n = get_twiddles(ctx, 2 * M, M, &rc, noalloc);  // W_N^k, k < M

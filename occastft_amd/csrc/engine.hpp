@@ -62,13 +62,17 @@ int fused_group_frames(int log_m);
 // the hop exceeds the window, so that the tile's span (tf - 1) hop + N fits
 // the G N samples of the staging buffer
 uint64_t fused_tile_frames(uint64_t G, uint64_t N, uint64_t hop);
-// Variants of the FP32 engine (Cfg<LOG_M, V> in stockham.hpp): bit 0 the
-// radix plan (1: the remainder radix first), bits 1-2 the workgroup shape of
-// the batch kernels (0: 256 threads or one transform, 1: 512, 2: 128).  Only
-// variants that differ from variant 0 are built: a plan of 1 where the size
-// has a remainder radix, shapes that change the workgroup.
-constexpr int FUSED_VARIANTS = 6;
+// Variants of the FP32 engine (Cfg<LOG_M, V> in stockham.hpp), V = 4 shape +
+// plan.  Plan bit 1: radix-8 passes with 8 values per thread (complex sizes
+// 2^7..2^13), else radix 16 with 16; bit 0: the remainder radix first.  Shape:
+// the workgroup of the batch kernels (0: 256 threads or one transform, 1: 512,
+// 2: 128).  Only variants that differ from the others are built: a
+// remainder-first plan where the size has a remainder radix, the radix-8
+// plans in shape 0, shapes that change the workgroup.
+constexpr int FUSED_VARIANTS = 12;
 bool fused_variant_ok(int log_m, int variant);
+// frames per workgroup (G) of variant `variant`, 0 if not built
+int fused_variant_frames(int log_m, int variant);
 bool fused_plan_ok(int log_m, int plan);
 // `count` signals (grid.y), signal y at a.in + y in_stride / a.out + y out_stride
 hipError_t launch_fused(int log_m, int xf, const FusedArgs& a, uint32_t count, hipStream_t s);

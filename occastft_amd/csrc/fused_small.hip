@@ -157,7 +157,7 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
                                              const uint32_t tid)
 {
     using C = Cfg<LOG_M, V>;
-    constexpr int M = C::M, N = 2 * M, T = C::T, G = C::G, WG = C::WG, PADM = C::PADM;
+    constexpr int M = C::M, N = 2 * M, T = C::T, G = C::G, WG = C::WG, PADM = C::PADM, VPT = C::VPT;
     float* sf = reinterpret_cast<float*>(lds);
 
     const uint32_t grp = tid / T, t = tid % T;
@@ -191,7 +191,7 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
         const uint64_t s0 = (f0 + grp) * hop;
         const bool pair_ok = ((reinterpret_cast<uintptr_t>(in + s0)) & 7u) == 0;
 #pragma unroll
-        for (int m = 0; m < 16; ++m) {
+        for (int m = 0; m < VPT; ++m) {
             const uint64_t s = s0 + 2ull * (t + m * T);
             xd[m] = make_float2(0.0f, 0.0f);
             if (active) {
@@ -234,7 +234,7 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
     float2 wv[16];
     if (w2) {
 #pragma unroll
-        for (int m = 0; m < 16; ++m)
+        for (int m = 0; m < VPT; ++m)
             wv[m] = w2[t + m * T];
     }
     PassTwiddles<LOG_M, V> ptw;
@@ -260,7 +260,7 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
     // flattened [frame][bin] index)
     constexpr int OITERS = (G * (M / 2 + 1) + WG - 1) / WG;
     // From T >= 16 (windows from 512) the N/2+1 layouts are written frame by
-    // frame: slots k < M/2 of a frame as k = t + j T (j < 8) by its own group
+    // frame: slots k < M/2 of a frame as k = t + j T (j < VPT/2) by its own group
     // when a group spans whole waves (T >= 64), else as k = lane + 64 jj
     // (jj < T/8) by a whole wave, frame after frame; the middle slot k = M/2
     // apart.  No per-slot index division, every address a per-thread base
@@ -271,7 +271,7 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
     // slot twiddles are loaded after the FFT, which keeps 18 registers free
     // through it
     constexpr bool SLOT_LATE_TW = SLOT_FRAME && T >= 512;
-    static_assert(!SLOT_FRAME || OITERS >= 9, "slot twiddles");
+    static_assert(!SLOT_FRAME || OITERS >= VPT / 2 + 1, "slot twiddles");
     constexpr int OITERP = (G * (M + 1) + 2 * WG - 1) / (2 * WG);
     constexpr int OITERM = OITERS > OITERP ? OITERS : OITERP;
     constexpr int NEB = OITERM > PITER ? (OITERM > OITER4 ? OITERM : OITER4) : (PITER > OITER4 ? PITER : OITER4);
@@ -306,15 +306,15 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
         if constexpr (SLOT_WAVE) {
             // the slot of lane l and step jj is the same in every frame
 #pragma unroll
-            for (int jj = 0; jj < T / 8; ++jj)
+            for (int jj = 0; jj < VPT * T / 128; ++jj)
                 eb[jj] = twn[(tid & 63u) + 64 * jj];
-            eb[T / 8] = twn[M / 2];
+            eb[VPT * T / 128] = twn[M / 2];
         } else if constexpr (SLOT_FRAME) {
             if constexpr (!SLOT_LATE_TW) {
 #pragma unroll
-                for (int j = 0; j < 8; ++j)
+                for (int j = 0; j < VPT / 2; ++j)
                     eb[j] = twn[t + j * T];
-                eb[8] = twn[M / 2];
+                eb[VPT / 2] = twn[M / 2];
             }
         } else if constexpr (SYM_REAL) {
 #pragma unroll
@@ -344,7 +344,7 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
     float2 v[16];
     if constexpr (DIRECT) {
 #pragma unroll
-        for (int m = 0; m < 16; ++m)
+        for (int m = 0; m < VPT; ++m)
             v[m] = w2 ? make_float2(xd[m].x * wv[m].x, xd[m].y * wv[m].y) : xd[m];
     } else {
         // stage the span in LDS; 4*WG is a multiple of 128, so vector i and
@@ -365,10 +365,10 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
         // KB = 64/T) or m (2T), a multiple of 128 (T >= 64, KB = 1), so it is
         // staged at spad(q0 + 2t + 2T b) + a ASTEP -- KB base addresses and
         // immediate offsets instead of one padded index per value
-        constexpr int KB = T >= 64 ? 1 : (64 / T < 16 ? 64 / T : 16);
+        constexpr int KB = T >= 64 ? 1 : (64 / T < VPT ? 64 / T : VPT);
         constexpr uint32_t ASTEP = T >= 64 ? 2 * T + T / 16 : 132;
 #pragma unroll
-        for (int m = 0; m < 16; ++m)
+        for (int m = 0; m < VPT; ++m)
             v[m] = make_float2(0.0f, 0.0f);
         if (active) {
             const uint32_t qt = q0 + 2 * t;
@@ -376,7 +376,7 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
                 cfor<0, KB>([&](auto bb) {
                     constexpr int B = decltype(bb)::value;
                     const float* const base = sf + spad(qt + 2 * T * B);
-                    cfor<0, 16 / KB>([&](auto aa) {
+                    cfor<0, VPT / KB>([&](auto aa) {
                         constexpr int A = decltype(aa)::value;
                         v[KB * A + B] = *reinterpret_cast<const float2*>(base + A * ASTEP);
                     });
@@ -386,7 +386,7 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
                     constexpr int B = decltype(bb)::value;
                     const float* const b0 = sf + spad(qt + 2 * T * B);
                     const float* const b1 = sf + spad(qt + 2 * T * B + 1);
-                    cfor<0, 16 / KB>([&](auto aa) {
+                    cfor<0, VPT / KB>([&](auto aa) {
                         constexpr int A = decltype(aa)::value;
                         v[KB * A + B] = make_float2(b0[A * ASTEP], b1[A * ASTEP]);
                     });
@@ -395,7 +395,7 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
         }
         if (w2) {
 #pragma unroll
-            for (int m = 0; m < 16; ++m)
+            for (int m = 0; m < VPT; ++m)
                 v[m] = make_float2(v[m].x * wv[m].x, v[m].y * wv[m].y);
         }
         __syncthreads();  // the FFT reuses the staging buffer
@@ -410,12 +410,12 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
     if (!FREQ_DC && a.dc) {
         float2 s = make_float2(0.0f, 0.0f);
 #pragma unroll
-        for (int m = 0; m < 16; ++m)
+        for (int m = 0; m < VPT; ++m)
             s = cadd(s, v[m]);
         s = group_sum<T, WG>(s, red, tid);
         const float mean = (s.x + s.y) / (float)N;
 #pragma unroll
-        for (int m = 0; m < 16; ++m)
+        for (int m = 0; m < VPT; ++m)
             v[m] = make_float2(v[m].x - mean, v[m].y - mean);
     }
 
@@ -553,7 +553,7 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
             // wave, one after the other (slots as in the SYM_REAL branch
             // below).  pad(l + 64 jj) = pad(l) + 68 jj and pad(M - 64 jj - l)
             // = PADM - 68 jj - (l + ceil(l/16)).
-            constexpr uint32_t FW = 64 / T, JW = T / 8;
+            constexpr uint32_t FW = 64 / T, JW = VPT * T / 128;
             const uint32_t lane = tid & 63u, wv = tid >> 6;
             const uint32_t lu = lane + ((lane + 15u) >> 4);
 #pragma unroll
@@ -602,21 +602,21 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
             if constexpr (SLOT_LATE_TW) {
                 if (COMB) {
 #pragma unroll
-                    for (int j = 0; j < 8; ++j)
+                    for (int j = 0; j < VPT / 2; ++j)
                         eb[j] = twn[t + j * T];
-                    eb[8] = twn[M / 2];
+                    eb[VPT / 2] = twn[M / 2];
                 }
             }
             if (active) {
                 constexpr uint32_t PT = T + T / 16;  // padded stride of j
                 const float2* z = lds + grp * PADM;
                 const float2* zl = z + pad(t);
-                const float2* zh = z + (PADM - 7 * PT) - (t + ((t + 15u) >> 4));  // + (7 - j) PT
+                const float2* zh = z + (PADM - (VPT / 2 - 1) * PT) - (t + ((t + 15u) >> 4));  // + (VPT/2 - 1 - j) PT
                 const uint64_t o = (f0 + grp) * (uint64_t)(M + 1);
 #pragma unroll
-                for (int j = 0; j < 8; ++j) {
+                for (int j = 0; j < VPT / 2; ++j) {
                     const uint32_t k = t + j * T;
-                    const float2 zm = (j == 0 && t == 0) ? z[0] : zh[(7 - j) * PT];
+                    const float2 zm = (j == 0 && t == 0) ? z[0] : zh[(VPT / 2 - 1 - j) * PT];
                     const BinPair b = finish_pair<COMB>(zl[j * PT], zm, eb[j]);
                     const float2 x2 = (COMB || k == 0) ? b.hi : b.lo;  // position M-k, before conj
                     if constexpr (MODE == OUT_POWER) {
@@ -632,7 +632,7 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
                     }
                 }
                 if (t == 0) {  // slot M/2: its own mirror
-                    const BinPair b = finish_pair<COMB>(z[pad(M / 2)], z[pad(M / 2)], eb[8]);
+                    const BinPair b = finish_pair<COMB>(z[pad(M / 2)], z[pad(M / 2)], eb[VPT / 2]);
                     if constexpr (MODE == OUT_POWER)
                         half_store(power_of(b.lo), out + o + M / 2);
                     else
@@ -855,24 +855,30 @@ static void host_for(F&& f)
     }
 }
 
-// variant validity (see FUSED_VARIANTS in engine.hpp): a plan of 1 only where
-// the size has a remainder radix; workgroup shapes only where they change the
-// workgroup (512 threads while a transform has fewer, 128 while it has at
-// most 128)
+// variant validity (see FUSED_VARIANTS in engine.hpp): the remainder-first
+// plans only where the size has a remainder radix; the radix-8 plans for
+// complex sizes 2^7..2^13 in the default workgroup shape; the other shapes
+// only where they change the workgroup (512 threads while a transform has
+// fewer, 128 while it has fewer than 256)
 template <int LOG_M, int V>
 constexpr bool variant_ok()
 {
-    constexpr int PLAN = V & 1, SHAPE = V >> 1;
-    return V >= 0 && V < FUSED_VARIANTS && !(PLAN && LOG_M % 4 == 0) && !(SHAPE == 1 && LOG_M > 12) &&
-           !(SHAPE == 2 && LOG_M > 11);
+    if constexpr (V < 0 || V >= FUSED_VARIANTS) {
+        return false;
+    } else {
+        using C = Cfg<LOG_M, V>;
+        constexpr int PLAN = V & 3, SHAPE = V >> 2;
+        return SHAPE <= 2 && !((PLAN & 2) && (LOG_M < 7 || LOG_M > 13 || SHAPE != 0)) &&
+               !((PLAN & 1) && C::REM == 0) && !(SHAPE == 1 && C::T >= 512) && !(SHAPE == 2 && C::T >= 256);
+    }
 }
 
 // f(integral_constant<int, V>) for the runtime variant v, if it is built
-template <int LOG_M, typename F>
+template <int LOG_M, int NV = FUSED_VARIANTS, typename F>
 static hipError_t with_variant(int v, F&& f)
 {
     hipError_t e = hipErrorInvalidValue;
-    host_for<0, FUSED_VARIANTS>([&](auto vc) {
+    host_for<0, NV>([&](auto vc) {
         constexpr int V = decltype(vc)::value;
         if constexpr (variant_ok<LOG_M, V>()) {
             if (v == V)
@@ -882,25 +888,25 @@ static hipError_t with_variant(int v, F&& f)
     return e;
 }
 
+// the one-request kernel runs the request's radix plan in the default shape
 template <int LOG_M>
 static hipError_t launch_t(const FusedArgs& a, uint32_t count, hipStream_t s)
 {
-    using C = Cfg<LOG_M>;  // both plans share the workgroup
-    if (a.tile_frames < 1 || a.tile_frames > C::G || (a.plan != 0 && a.plan != 1))
+    if (a.plan < 0 || a.plan > 3)
         return hipErrorInvalidValue;
-    const uint64_t blocks = (a.frames + a.tile_frames - 1) / a.tile_frames;
-    if (blocks == 0 || count == 0)
-        return hipSuccess;
-    if (blocks > 0x7fffffffull || count > 65535u)
-        return hipErrorInvalidValue;
-    if constexpr (variant_ok<LOG_M, 1>()) {
-        if (a.plan == 1) {
-            hipLaunchKernelGGL((stft_fused_kernel<LOG_M, 1>), dim3((uint32_t)blocks, count), dim3(C::WG), 0, s, a);
-            return hipGetLastError();
-        }
-    }
-    hipLaunchKernelGGL((stft_fused_kernel<LOG_M, 0>), dim3((uint32_t)blocks, count), dim3(C::WG), 0, s, a);
-    return hipGetLastError();
+    return with_variant<LOG_M, 4>(a.plan, [&](auto pc) -> hipError_t {
+        constexpr int P = decltype(pc)::value;
+        using C = Cfg<LOG_M, P>;
+        if (a.tile_frames < 1 || a.tile_frames > C::G)
+            return hipErrorInvalidValue;
+        const uint64_t blocks = (a.frames + a.tile_frames - 1) / a.tile_frames;
+        if (blocks == 0 || count == 0)
+            return hipSuccess;
+        if (blocks > 0x7fffffffull || count > 65535u)
+            return hipErrorInvalidValue;
+        hipLaunchKernelGGL((stft_fused_kernel<LOG_M, P>), dim3((uint32_t)blocks, count), dim3(C::WG), 0, s, a);
+        return hipGetLastError();
+    });
 }
 
 template <int LOG_M>
@@ -1016,7 +1022,22 @@ bool fused_variant_ok(int log_m, int variant)
     });
 }
 
-bool fused_plan_ok(int log_m, int plan) { return plan == 0 || (plan == 1 && fused_variant_ok(log_m, 1)); }
+bool fused_plan_ok(int log_m, int plan) { return plan >= 0 && plan < 4 && fused_variant_ok(log_m, plan); }
+
+int fused_variant_frames(int log_m, int variant)
+{
+    return with_log_m(log_m, 0, [&](auto lc) {
+        int g = 0;
+        host_for<0, FUSED_VARIANTS>([&](auto vc) {
+            constexpr int L = decltype(lc)::value, V = decltype(vc)::value;
+            if constexpr (variant_ok<L, V>()) {
+                if (variant == V)
+                    g = Cfg<L, V>::G;
+            }
+        });
+        return g;
+    });
+}
 
 hipError_t launch_batch(int log_m, int variant, const BatchClassArgs& b, uint32_t grid, bool persistent,
                         hipStream_t s)

@@ -1,10 +1,11 @@
 // stockham.hpp -- the LDS-exchanged Stockham engine shared by the fused
 // single-kernel path and the multi-pass column kernels.
 //
-// A "group" of T = M/16 threads owns one complex transform of size M = 2^LOG_M
-// with 16 values per thread in VGPRs.  Passes are radix 16 (plus one final
-// radix 2/4/8 pass); pass PASS with radix r and stride Ns (product of the
-// earlier radices) does, for every butterfly g < M/r,
+// A "group" of T = M/VPT threads owns one complex transform of size
+// M = 2^LOG_M with VPT = 16 (or 8) values per thread in VGPRs.  Passes are
+// radix VPT plus one radix 2/4/8 pass for the remainder; pass PASS with radix
+// r and stride Ns (product of the earlier radices) does, for every butterfly
+// g < M/r,
 //     v[m] = X[g + m*M/r] * W_{Ns*r}^{(g mod Ns)*m},   v = DFT_r(v),
 //     Y[(g/Ns)*Ns*r + (g mod Ns) + q*Ns] = v[q],
 // which leaves the spectrum in natural order (Stockham autosort).
@@ -30,28 +31,31 @@
 namespace ostft {
 
 // Shape and radix plan of the engine for complex size M = 2^LOG_M.  Variant
-// V: bit 0 is the radix plan (0: the radix-16 passes, then the remainder radix
-// 2/4/8; 1: the remainder radix first), bits 1-2 the workgroup (0: at least
-// OSTFT_WG_MIN threads, 1: at least 512, 2: at least 128).  A transform's
-// arithmetic depends on the plan only; the workgroup decides how many
-// transforms share one.
+// V = 4 SHAPE + PLAN.  PLAN bit 1: radix-8 passes with 8 values per thread
+// (twice the threads per transform, about half the registers per thread),
+// else radix-16 passes with 16; bit 0: the remainder radix first, else last.
+// SHAPE: the workgroup (0: at least OSTFT_WG_MIN threads, 1: at least 512,
+// 2: at least 128).  A transform's arithmetic depends on the plan only; the
+// workgroup decides how many transforms share one.
 template <int LOG_M, int V = 0>
 struct Cfg {
     static constexpr int M = 1 << LOG_M;
-    static constexpr int T = M / 16;               // threads per transform
-    static constexpr int PLAN = V & 1, SHAPE = V >> 1;
+    static constexpr int PLAN = V & 3, SHAPE = V >> 2;
+    static constexpr int VPT = (PLAN & 2) ? 8 : 16;  // values per thread
+    static constexpr int LV = VPT == 16 ? 4 : 3;
+    static constexpr int T = M / VPT;              // threads per transform
     static constexpr int WG_MIN = SHAPE == 1   ? 512
                                   : SHAPE == 2 ? 128
                                                : (LOG_M >= OSTFT_BIG_LOG_M ? OSTFT_WG_BIG : OSTFT_WG_MIN);
     static constexpr int WG = T < WG_MIN ? WG_MIN : T;  // threads per workgroup
     static constexpr int G = WG / T;               // transforms per workgroup
     static constexpr int PADM = M + M / 16;        // padded LDS length per transform
-    static constexpr int NP16 = LOG_M / 4;
-    static constexpr int REM = LOG_M % 4;
-    static constexpr int NPASS = NP16 + (REM ? 1 : 0);
+    static constexpr int NPV = LOG_M / LV;         // radix-VPT passes
+    static constexpr int REM = LOG_M % LV;
+    static constexpr int NPASS = NPV + (REM ? 1 : 0);
     static constexpr int radix(int i)
     {
-        return (PLAN && REM) ? (i == 0 ? (1 << REM) : 16) : (i < NP16 ? 16 : (1 << REM));
+        return ((PLAN & 1) && REM) ? (i == 0 ? (1 << REM) : VPT) : (i < NPV ? VPT : (1 << REM));
     }
     static constexpr int ns(int i)
     {
@@ -142,7 +146,7 @@ __device__ __forceinline__ float2 group_sum(float2 s, float2* red, uint32_t tid)
 // against 0.24 ms at 4096 with 50% overlap.)
 template <int LOG_M, int V = 0>
 struct PassTwiddles {
-    float2 b[Cfg<LOG_M, V>::NPASS][16][3];  // [pass][butterfly u][w, w^4, w^8]; unused slots vanish
+    float2 b[Cfg<LOG_M, V>::NPASS][Cfg<LOG_M, V>::VPT][3];  // [pass][butterfly u][w, w^4, w^8]; unused slots vanish
 };
 
 template <int LOG_M, int V = 0>
@@ -153,7 +157,7 @@ __device__ __forceinline__ void load_pass_twiddles(PassTwiddles<LOG_M, V>& P, co
     constexpr int M = C::M, T = C::T;
     cfor<1, C::NPASS>([&](auto p) {
         constexpr int PP = decltype(p)::value;
-        constexpr int r = C::radix(PP), NS = C::ns(PP), U = 16 / r, STEP = M / (NS * r);
+        constexpr int r = C::radix(PP), NS = C::ns(PP), U = C::VPT / r, STEP = M / (NS * r);
         cfor<0, U>([&](auto u) {
             constexpr int UU = decltype(u)::value;
             const uint32_t e1 = ((t + UU * T) & (NS - 1)) * STEP;
@@ -168,13 +172,14 @@ __device__ __forceinline__ void load_pass_twiddles(PassTwiddles<LOG_M, V>& P, co
 
 // The first pass expects, for butterfly u of the thread, v[u r + m] =
 // X[t + u T + m M/r]; the gather delivers v[j] = X[t + j T].  For a first
-// radix below 16 (plan 1) that is a fixed renaming of the 16 registers.
+// radix below VPT (the remainder first) that is a fixed renaming of the
+// thread's registers.
 template <int LOG_M, int V = 0>
 __device__ __forceinline__ void first_pass_order(float2 (&v)[16])
 {
     using C = Cfg<LOG_M, V>;
-    constexpr int r = C::radix(0), U = 16 / r;
-    if constexpr (r != 16) {
+    constexpr int r = C::radix(0), U = C::VPT / r;
+    if constexpr (r != C::VPT) {
         float2 w[16];
         cfor<0, U>([&](auto u) {
             constexpr int UU = decltype(u)::value;
@@ -183,7 +188,7 @@ __device__ __forceinline__ void first_pass_order(float2 (&v)[16])
                 w[UU * r + MM] = v[UU + MM * U];  // X[t + (u + m U) T], m U T = m M/r
             });
         });
-        cfor<0, 16>([&](auto j) { v[decltype(j)::value] = w[decltype(j)::value]; });
+        cfor<0, C::VPT>([&](auto j) { v[decltype(j)::value] = w[decltype(j)::value]; });
     }
 }
 
@@ -195,7 +200,7 @@ __device__ __forceinline__ void stockham(float2 (&v)[16], float2* z, const PassT
 {
     using C = Cfg<LOG_M, V>;
     constexpr int M = C::M, T = C::T;
-    constexpr int r = C::radix(PASS), NS = C::ns(PASS), U = 16 / r;
+    constexpr int r = C::radix(PASS), NS = C::ns(PASS), U = C::VPT / r;
     if constexpr (NS > 1) {
         cfor<0, U>([&](auto u) {
             constexpr int UU = decltype(u)::value;
@@ -228,24 +233,31 @@ __device__ __forceinline__ void stockham(float2 (&v)[16], float2* z, const PassT
         constexpr int UU = decltype(u)::value;
         dft<r, UU * r>(v);
     });
+    // pad(base + q NS) = pad(base) + pad(q NS): base = B + s with B a
+    // multiple of 16 (when NS r >= 16) and s < NS, and s + (q NS mod 16) < 16
+    // when NS < 16; base = g r and q < r when NS r < 16.  Likewise
+    // pad(g + m S) = pad(g) + pad(m S) for S = M / r2 (a multiple of 16, or
+    // g < S with S a divisor of 16).  So each butterfly needs one padded base
+    // and the rest are constant offsets.
     cfor<0, U>([&](auto u) {
         constexpr int UU = decltype(u)::value;
         const uint32_t g = t + UU * T;
         const uint32_t base = (g / NS) * (NS * r) + (g & (NS - 1));
+        float2* const zb = z + pad(base);
         cfor<0, r>([&](auto q) {
             constexpr int QQ = decltype(q)::value;
-            z[pad(base + QQ * NS)] = v[UU * r + QQ];
+            zb[pad(QQ * NS)] = v[UU * r + QQ];
         });
     });
     __syncthreads();
     if constexpr (PASS + 1 < C::NPASS) {
-        constexpr int r2 = C::radix(PASS + 1), U2 = 16 / r2;
+        constexpr int r2 = C::radix(PASS + 1), U2 = C::VPT / r2;
         cfor<0, U2>([&](auto u) {
             constexpr int UU = decltype(u)::value;
-            const uint32_t g = t + UU * T;
+            const float2* const zg = z + pad(t + UU * T);
             cfor<0, r2>([&](auto m) {
                 constexpr int MM = decltype(m)::value;
-                v[UU * r2 + MM] = z[pad(g + MM * (M / r2))];
+                v[UU * r2 + MM] = zg[pad(MM * (M / r2))];
             });
         });
         __syncthreads();

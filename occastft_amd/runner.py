@@ -213,10 +213,12 @@ class Runner:
     def engine_variant(self, exp: int, shape: int = -1, plan: int = -1):
         """-> (shape, plan) in force for window exponent `exp` (6..15), after
         setting either (-1 leaves it).  plan: radix plan of the FP32 engine
-        for every request of that window on this context (0: radix-16 passes
-        first, 1: the remainder radix first; exponents 9 and 13 have plan 0
-        only).  shape: frames per workgroup of the batch kernels (0: 256
-        threads, 1: 512, 2: 128; shapes 1 and 2 run one workgroup per tile)."""
+        for every request of that window on this context (0: radix-16 passes,
+        remainder radix last; 1: remainder first; 2, 3: the same with radix-8
+        passes, exponents 8..14; plans 1 and 3 only where the size has a
+        remainder radix).  shape: frames per workgroup of the batch kernels
+        (0: 256 threads, 1: 512, 2: 128; plans 2 and 3 shape 0 only; shapes 1
+        and 2 run one workgroup per tile)."""
         cs = ctypes.c_int32()
         cp = ctypes.c_int32()
         rc = N.lib().ostft_engine_variant(self._h, int(exp), int(shape), int(plan), ctypes.byref(cs),

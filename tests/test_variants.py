@@ -2,12 +2,15 @@
 of the FP32 single-kernel transform and the workgroup shape of the batch
 kernels.
 
+Plans: 0 radix-16 passes, remainder radix last; 1 remainder first; 2 radix-8
+passes (8 values per thread), remainder last; 3 radix 8, remainder first.
+
 Bars:
-  * a radix plan is the transform's arithmetic: plan 1 (the remainder radix
-    first) is checked against the oracle at the parity suite's bar (per-frame
+  * a radix plan is the transform's arithmetic: every built plan other than 0
+    is checked against the oracle at the parity suite's bar (per-frame
     relative L2 <= 1e-5 against the float32 restatement of the reference
     Serial runner; <= 1e-5 against the FP64 pipeline for the new layouts), at
-    every window whose size has a remainder radix;
+    every window it is built for;
   * a workgroup shape only groups frames: under every (shape, plan) every
     request of a ragged batch is bitwise identical to the same request run
     alone with ostft_run_device on the same context (which uses the same
@@ -27,11 +30,25 @@ pytestmark = pytest.mark.gpu
 
 TOL = 1e-5
 EXPS = range(6, 16)                     # window exponents of the batch kernels
-PLAN1 = [e for e in EXPS if (e - 1) % 4]  # sizes with a remainder radix: all but 9 and 13
 
 
-def shapes_of(exp):
+def plans_of(exp):
+    """Built radix plans of window 2^exp (complex size 2^(exp-1))."""
+    lm = exp - 1
+    plans = [0] + ([1] if lm % 4 else [])
+    if 7 <= lm <= 13:
+        plans += [2] + ([3] if lm % 3 else [])
+    return plans
+
+
+def shapes_of(exp, plan=0):
+    if plan >= 2:
+        return [0]
     return [0, 1] + ([2] if exp <= 12 else []) if exp <= 13 else [0]
+
+
+PLAN1 = [e for e in EXPS if 1 in plans_of(e)]  # sizes with a remainder radix: all but 9 and 13
+OTHER_PLANS = [(e, p) for e in EXPS for p in plans_of(e) if p]
 
 
 @pytest.fixture(scope="module")
@@ -65,13 +82,14 @@ def bitwise_equal(a, b):
 def test_variant_validity(runner):
     for e in EXPS:
         assert runner.engine_variant(e) == (0, 0)
-        for plan in (0, 1):
-            if plan == 1 and e not in PLAN1:
+        for plan in range(5):
+            if plan not in plans_of(e):
                 with pytest.raises(ostft.STFTError) as ei:
                     runner.engine_variant(e, plan=plan)
                 assert ei.value.status == N.ERR_UNSUPPORTED
             else:
                 assert runner.engine_variant(e, plan=plan)[1] == plan
+        runner.engine_variant(e, 0, 0)
         for shape in range(4):
             if shape in shapes_of(e):
                 assert runner.engine_variant(e, shape=shape)[0] == shape
@@ -84,6 +102,14 @@ def test_variant_validity(runner):
         with pytest.raises(ostft.STFTError) as ei:
             runner.engine_variant(e, plan=0)
         assert ei.value.status == N.ERR_UNSUPPORTED
+    # the radix-8 plans have the default shape only
+    with pytest.raises(ostft.STFTError) as ei:
+        runner.engine_variant(10, shape=1, plan=2)
+    assert ei.value.status == N.ERR_UNSUPPORTED
+    runner.engine_variant(10, shape=1)
+    with pytest.raises(ostft.STFTError):
+        runner.engine_variant(10, plan=2)
+    runner.engine_variant(10, 0, 0)
     # the persistent form has shape 0 only: selecting it resets the shape
     runner.engine_variant(10, shape=1)
     assert runner.batch_config(10, blocks_per_cu=1) == 1
@@ -93,21 +119,21 @@ def test_variant_validity(runner):
     assert runner.batch_config(10) == 0
 
 
-@pytest.mark.parametrize("exp", PLAN1)
-def test_plan1_against_oracle(runner, exp):
+@pytest.mark.parametrize("exp,plan", OTHER_PLANS)
+def test_plan_against_oracle(runner, exp, plan):
     rng = np.random.default_rng(7000 + exp)
     N_ = 1 << exp
     x = rng.uniform(-1, 1, 9 * N_ + 333).astype(np.float32)
     cases = [(0.5, "--hanning_window --remove_dc"), (0.75, "--blackman_window --half_complex_return"),
              (0.0, "--remove_dc")]
     base = {}
-    for plan in (0, 1):
-        runner.engine_variant(exp, plan=plan)
+    for pl in (0, plan):
+        runner.engine_variant(exp, plan=pl)
         for ov, opts in cases:
             y = runner.stft(x, exp, ov, opts)
             rc, r = po.pipeline(x, exp, ov, opts)
             assert rc == 0 and y.shape == r.shape
-            check(y, r, N_, f"plan {plan} exp={exp} ov={ov} opts='{opts}'")
+            check(y, r, N_, f"plan {pl} exp={exp} ov={ov} opts='{opts}'")
             base.setdefault((ov, opts), []).append(y)
         # a new layout against the FP64 pipeline
         opts = "--hanning_window --remove_dc --exact_window --exact_odd_exp --power_return"
@@ -115,22 +141,23 @@ def test_plan1_against_oracle(runner, exp):
         g = ostft.geometry(ostft.parse_options(opts, exp, 0.5), len(x))
         t = po.stft64(x, exp, g["hop"], g["frames"], po.WINDOWS["hanning"], 0.0, True, po.OUT_POWER,
                       quirk_odd=False)
-        check(y, t, N_ // 2 + 1, f"plan {plan} exp={exp} power vs fp64")
+        check(y, t, N_ // 2 + 1, f"plan {pl} exp={exp} power vs fp64")
     # the plans are different arithmetic: they agree to rounding, and the
-    # plan-1 kernel did run (its results are not bit for bit plan 0's)
-    assert any(not bitwise_equal(a, b) for a, b in base.values()), f"exp {exp}: plan 1 gave plan 0's bits"
+    # plan's kernel did run (its results are not bit for bit plan 0's)
+    assert any(not bitwise_equal(a, b) for a, b in base.values()), f"exp {exp}: plan {plan} gave plan 0's bits"
 
 
 @pytest.mark.parametrize("shape,plan,form", [(0, 1, "dispatch"), (0, 1, "persistent"), (1, 0, "dispatch"),
-                                             (1, 1, "dispatch"), (2, 0, "dispatch"), (2, 1, "dispatch")])
+                                             (1, 1, "dispatch"), (2, 0, "dispatch"), (2, 1, "dispatch"),
+                                             (0, 2, "dispatch"), (0, 3, "persistent")])
 def test_batch_variant_matches_requests_run_alone(runner, shape, plan, form):
     import torch
 
     from test_batch_serving import device_inputs, ragged_requests, run_alone
 
     for e in EXPS:
-        runner.engine_variant(e, shape=shape if shape in shapes_of(e) else 0,
-                              plan=plan if e in PLAN1 else 0)
+        pl = plan if plan in plans_of(e) else (plan & 1 if (plan & 1) in plans_of(e) else 0)
+        runner.engine_variant(e, shape=shape if shape in shapes_of(e, pl) else 0, plan=pl)
         if form == "persistent":
             runner.batch_config(e, blocks_per_cu=2)
     reqs = ragged_requests(100 + 10 * shape + plan, 48, exps=list(EXPS))
@@ -142,13 +169,13 @@ def test_batch_variant_matches_requests_run_alone(runner, shape, plan, form):
         assert bitwise_equal(o.cpu().numpy(), a), f"request {i}: exp {reqs[i][1].exp} out {reqs[i][1].output}"
 
 
-@pytest.mark.parametrize("exp", [6, 8, 11])
-def test_plan1_any_hop(runner, exp):
-    """Plan 1 with an explicit geometry: hop 1, a hop past the window (fewer
-    frames per tile), against the FP64 pipeline."""
+@pytest.mark.parametrize("exp,plan", [(6, 1), (8, 1), (11, 1), (8, 2), (11, 3), (14, 2)])
+def test_plan_any_hop(runner, exp, plan):
+    """A plan other than 0 with an explicit geometry: hop 1, a hop past the
+    window (fewer frames per tile), against the FP64 pipeline."""
     from test_mfma import _activate_geometry
 
-    runner.engine_variant(exp, plan=1)
+    runner.engine_variant(exp, plan=plan)
     N_ = 1 << exp
     rng = np.random.default_rng(exp)
     for hop in (1, N_ // 2 + 3, 3 * N_ + 5):
@@ -170,7 +197,7 @@ def test_autotune_picks_a_variant_and_keeps_results(runner):
         assert choice.ms > 0 and choice.blocks_per_cu >= 0
         assert runner.engine_variant(e) == (choice.shape, choice.plan)
         assert runner.batch_config(e) == choice.blocks_per_cu
-        assert choice.shape in shapes_of(e) and (choice.plan == 0 or e in PLAN1)
+        assert choice.plan in plans_of(e) and choice.shape in shapes_of(e, choice.plan)
         assert choice.blocks_per_cu == 0 or choice.shape == 0
         reqs = ragged_requests(50 + e, 10, exps=[e])
         alone = [run_alone(runner, x, p) for x, p in reqs]
