@@ -404,7 +404,7 @@ def bench_sweep(args, runner):
     b = algorithmic_bytes(g, 1 << e)
     cold_ms = cold_time(lambda: runner.run_device(x, p, out=out, stream=stream), stream, max(3, args.steps))
     total_bytes = sum(algorithmic_bytes(g, 1 << e) for e, ov, name, p, g in reqs)
-    kernel_key = f"stft_fused_kernel<{e - 1}> window {1 << e} overlap {ov} {name}"
+    kernel_key = f"stft_fused_kernel<{e - 1}, 0> window {1 << e} overlap {ov} {name}"
     res = header(args, frames / (ms_per_step / 1e3), "frames/s", ms_per_step, {
         "workload": "plain fused FP32 small-window sweep: 1e8-sample signal (8 sinusoids + N(0,0.05^2)), "
                     "windows 2^6..2^12, overlap 0/50%, Hanning + DC removal, power and half-complex "

@@ -164,7 +164,7 @@ def test_odd_exponent_result_buffer(runner, exp):
     check(y, t, N_, TOL, "exact odd")
 
 
-@pytest.mark.parametrize("exp", [6, 8, 10, 12, 13, 14, 16])
+@pytest.mark.parametrize("exp", [6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16])
 def test_new_output_layouts_against_fp64(runner, exp):
     rng = np.random.default_rng(4000 + exp)
     N_ = 1 << exp
