@@ -151,6 +151,13 @@ def test_context_calls_reject_a_null_context():
     assert cur.value == 7
     best = ctypes.c_int32()
     assert N.lib().ostft_batch_config(None, 10, 1, -1, ctypes.byref(best)) == N.ERR_INVALID_ARG
+    shape, plan = ctypes.c_int32(-5), ctypes.c_int32(-5)
+    assert N.lib().ostft_engine_variant(None, 10, 0, 0, ctypes.byref(shape), ctypes.byref(plan)) == N.ERR_INVALID_ARG
+    assert shape.value == -5 and plan.value == -5
+    ms = ctypes.c_double(-1.0)
+    assert N.lib().ostft_autotune_batch(None, 10, 1 << 20, ctypes.byref(best), ctypes.byref(shape),
+                                        ctypes.byref(plan), ctypes.byref(ms)) == N.ERR_INVALID_ARG
+    assert ms.value == -1.0
 
 
 def test_create_without_gpu_is_a_clean_error():
