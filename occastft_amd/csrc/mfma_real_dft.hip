@@ -632,10 +632,49 @@ __global__ void __launch_bounds__(64) mfma_rtable_kernel(const float* __restrict
             tt[n1 * 32 + j] = twn[n1 * (j == 0 ? 32u : j)];
 }
 
+// Lane q of each quad holds row q of a 4 x 4 matrix in a[0..3]; afterwards
+// it holds column q.  The off-diagonal 2 x 2 blocks are exchanged with the
+// lane two apart, then the off-diagonal elements of each block with the lane
+// one apart (quad permutations as DPP moves; each lane sends the values its
+// partner keeps).
+__device__ __forceinline__ float quad_xor1(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xB1, 0xF, 0xF, false));  // [1,0,3,2]
+}
+__device__ __forceinline__ float quad_xor2(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x4E, 0xF, 0xF, false));  // [2,3,0,1]
+}
+__device__ __forceinline__ void quad_transpose(float (&a)[4], uint32_t q)
+{
+    const bool hi = (q & 2u) != 0, odd = (q & 1u) != 0;
+    const float s0 = quad_xor2(hi ? a[0] : a[2]), s1 = quad_xor2(hi ? a[1] : a[3]);
+    if (hi) {
+        a[0] = s0;
+        a[1] = s1;
+    } else {
+        a[2] = s0;
+        a[3] = s1;
+    }
+    const float r0 = quad_xor1(odd ? a[0] : a[1]), r1 = quad_xor1(odd ? a[2] : a[3]);
+    if (odd) {
+        a[0] = r0;
+        a[2] = r1;
+    } else {
+        a[1] = r0;
+        a[3] = r1;
+    }
+}
+
 // this lane's samples of tile `tile`: x[ks][t] = x[f hop + n1 + R (16 ks + 8 h + t)]
 // (row (f, n1) of the GEMM), zeros past the signal, its frames or its tiles
-template <int R>
-__device__ __forceinline__ void load_rtile(const float* __restrict__ in, uint64_t L, uint64_t hop, uint64_t frames,
+// VEC (window 256 only; the caller checks that the input, the hop and the
+// signal stride are multiples of 4 floats): a tile inside the signal loads
+// 16-byte blocks, left for quad_transpose_tile where they are used (the
+// transposes' temporaries would not fit beside the tile in flight); the
+// return value says so
+template <int R, bool VEC = false>
+__device__ __forceinline__ bool load_rtile(const float* __restrict__ in, uint64_t L, uint64_t hop, uint64_t frames,
                                            uint64_t tiles, uint64_t tile, uint32_t n1r, uint32_t fr, uint32_t h,
                                            float (&x)[RKS][8])
 {
@@ -643,6 +682,26 @@ __device__ __forceinline__ void load_rtile(const float* __restrict__ in, uint64_
     const uint64_t f = tile * FPT + fr;
     // a tile wholly inside the signal (uniform over the wave) loads unguarded
     if (tile < tiles && (tile + 1) * FPT <= frames && ((tile + 1) * FPT - 1) * hop + N <= L) {
+        if constexpr (VEC && R == 4) {
+            // the quad's lanes (n1r = 0..3, one frame and h) read the 32
+            // consecutive samples of step ks together: lane q loads those of
+            // t = q and t = q + 4 as two 16-byte blocks (values n1r = 0..3)
+            const float* __restrict__ qsrc = in + f * hop + 32u * h;
+#pragma unroll
+            for (int ks = 0; ks < RKS; ++ks) {
+                const float4 u0 = *reinterpret_cast<const float4*>(qsrc + 4u * (16u * ks + n1r));
+                const float4 u1 = *reinterpret_cast<const float4*>(qsrc + 4u * (16u * ks + n1r + 4u));
+                x[ks][0] = u0.x;
+                x[ks][1] = u0.y;
+                x[ks][2] = u0.z;
+                x[ks][3] = u0.w;
+                x[ks][4] = u1.x;
+                x[ks][5] = u1.y;
+                x[ks][6] = u1.z;
+                x[ks][7] = u1.w;
+            }
+            return true;
+        }
         const float* __restrict__ src = in + f * hop + n1r + (uint64_t)R * 8u * h;
 #pragma unroll
         for (int ks = 0; ks < RKS; ++ks)
@@ -660,12 +719,13 @@ __device__ __forceinline__ void load_rtile(const float* __restrict__ in, uint64_
                 x[ks][t] = (row_ok && s < L) ? in[s] : 0.0f;
             }
     }
+    return false;
 }
 
 // Persistent: wave w of workgroup b takes tiles b W + w, then every
 // gridDim.x W-th one, so the table is copied to LDS once per workgroup; the
 // next tile's samples load while the current one's bins are stored.
-template <int LOG_R, int XF>
+template <int LOG_R, int XF, bool VEC>
 __global__ void __launch_bounds__(RCfg<LOG_R>::WAVES * 64) __attribute__((amdgpu_waves_per_eu(OSTFT_MFMA_RMINWAVES)))
 stft_mfma_r_kernel(const FusedArgs a, const uint32_t tiles_per_signal)
 {
@@ -692,7 +752,7 @@ stft_mfma_r_kernel(const FusedArgs a, const uint32_t tiles_per_signal)
     const uint64_t stride = (uint64_t)gridDim.x * W;
     uint64_t tile = (uint64_t)xcd_block(blockIdx.x, gridDim.x) * W + wave;
     float x[RKS][8];
-    load_rtile<R>(in, L, hop, frames, tiles_per_signal, tile, n1r0, fr0, h0, x);
+    bool raw = load_rtile<R, VEC>(in, L, hop, frames, tiles_per_signal, tile, n1r0, fr0, h0, x);
     {
         constexpr uint32_t NB = RB_BYTES / 16, NW = RW_BYTES / 16, NU = NB + NW + rt_bytes(R) / 16;
         const uint4* __restrict__ src = reinterpret_cast<const uint4*>(a.mfma_b);
@@ -719,6 +779,20 @@ stft_mfma_r_kernel(const FusedArgs a, const uint32_t tiles_per_signal)
         uint32_t lane = lane0, j = j0, h = h0, n1r = n1r0, fr = fr0;
         asm volatile("" : "+v"(lane), "+v"(j), "+v"(h), "+v"(n1r), "+v"(fr));
         const uint64_t f0 = tile * FPT;
+        if constexpr (VEC) {
+            if (raw) {  // 16-byte blocks of the quad: one transpose per block
+#pragma unroll
+                for (int ks = 0; ks < RKS; ++ks)
+#pragma unroll
+                    for (int b = 0; b < 2; ++b) {
+                        float q4[4] = {x[ks][4 * b], x[ks][4 * b + 1], x[ks][4 * b + 2], x[ks][4 * b + 3]};
+                        quad_transpose(q4, n1r);
+#pragma unroll
+                        for (int t = 0; t < 4; ++t)
+                            x[ks][4 * b + t] = q4[t];
+                    }
+            }
+        }
         // windowed samples and the frame's largest |value| (lanes R fr .. R fr + R - 1, both h)
         float mx = 0.0f;
 #pragma unroll
@@ -843,7 +917,7 @@ stft_mfma_r_kernel(const FusedArgs a, const uint32_t tiles_per_signal)
             }
         }
         // the next tile's samples load while this one's bins are stored
-        load_rtile<R>(in, L, hop, frames, tiles_per_signal, tile + stride, n1r, fr, h, x);
+        raw = load_rtile<R, VEC>(in, L, hop, frames, tiles_per_signal, tile + stride, n1r, fr, h, x);
         // slot q's bins to sink(k, re, im):
         //   lane j >= 1: X[j + 64 k1] = Z[k1], X[64 - j + 64 k1] = conj Z[R - 1 - k1];
         //   lane 0:      X[64 k1] = V[2 k1],   X[32 + 64 k1] = V[2 k1 + 1], X[N/2] = V[R]
@@ -928,8 +1002,15 @@ hipError_t launch_r(const FusedArgs& a, uint32_t count, hipStream_t s)
     const uint64_t total = (uint64_t)device_cus() * OSTFT_MFMA_GRID_WAVES_PER_CU / C::WAVES;
     const uint64_t cap = (total + count - 1) / count;
     const uint64_t blocks = need < cap ? need : cap;
-    hipLaunchKernelGGL((stft_mfma_r_kernel<LOG_R, XF>), dim3((uint32_t)blocks, count), dim3(C::WAVES * 64), 0, s,
-                       a, (uint32_t)tps);
+    // 16-byte input loads where every frame starts on a 16-byte boundary
+    const bool vec = LOG_R == 2 && (reinterpret_cast<uintptr_t>(a.in) & 15u) == 0 && (a.hop & 3u) == 0 &&
+                     (count == 1 || (a.in_stride & 3u) == 0);
+    if (vec)
+        hipLaunchKernelGGL((stft_mfma_r_kernel<LOG_R, XF, LOG_R == 2>), dim3((uint32_t)blocks, count),
+                           dim3(C::WAVES * 64), 0, s, a, (uint32_t)tps);
+    else
+        hipLaunchKernelGGL((stft_mfma_r_kernel<LOG_R, XF, false>), dim3((uint32_t)blocks, count),
+                           dim3(C::WAVES * 64), 0, s, a, (uint32_t)tps);
     return hipGetLastError();
 }
 
