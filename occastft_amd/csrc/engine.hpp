@@ -56,8 +56,6 @@ hipError_t launch_mfma_gemm(int log_m, int xf, const FusedArgs& a, uint32_t coun
 
 // Single-kernel path: complex size M = 2^log_m, 5 <= log_m <= 14 (N = 64 .. 32768).
 bool fused_supported(int log_m);
-// frames per workgroup (G) of the single kernel at complex size 2^log_m
-int fused_group_frames(int log_m);
 // frames per tile for a workgroup of G frames, window N and hop: G, fewer when
 // the hop exceeds the window, so that the tile's span (tf - 1) hop + N fits
 // the G N samples of the staging buffer

@@ -262,13 +262,13 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
     // From T >= 16 (windows from 512) the N/2+1 layouts are written frame by
     // frame: slots k < M/2 of a frame as k = t + j T (j < VPT/2) by its own group
     // when a group spans whole waves (T >= 64), else as k = lane + 64 jj
-    // (jj < T/8) by a whole wave, frame after frame; the middle slot k = M/2
+    // (jj < VPT T/128) by a whole wave, frame after frame; the middle slot k = M/2
     // apart.  No per-slot index division, every address a per-thread base
     // plus a constant, and each store instruction one contiguous run.
     constexpr bool SLOT_FRAME = SYM_REAL && T >= OSTFT_SLOT_MIN_T;
     constexpr bool SLOT_WAVE = SLOT_FRAME && T < 64;
     // at the largest transforms (one frame per workgroup, four passes) the
-    // slot twiddles are loaded after the FFT, which keeps 18 registers free
+    // slot twiddles are loaded after the FFT, which keeps their registers free
     // through it
     constexpr bool SLOT_LATE_TW = SLOT_FRAME && T >= 512;
     static_assert(!SLOT_FRAME || OITERS >= VPT / 2 + 1, "slot twiddles");
@@ -998,11 +998,6 @@ static R with_log_m(int log_m, R fail, F&& f)
             r = f(lc);
     });
     return r;
-}
-
-int fused_group_frames(int log_m)
-{
-    return with_log_m(log_m, 1, [](auto lc) { return Cfg<decltype(lc)::value>::G; });
 }
 
 uint64_t fused_tile_frames(uint64_t G, uint64_t N, uint64_t hop)
