@@ -169,10 +169,13 @@ def test_batch_variant_matches_requests_run_alone(runner, shape, plan, form):
         assert bitwise_equal(o.cpu().numpy(), a), f"request {i}: exp {reqs[i][1].exp} out {reqs[i][1].output}"
 
 
-@pytest.mark.parametrize("exp,plan", [(6, 1), (8, 1), (11, 1), (8, 2), (11, 3), (14, 2)])
+@pytest.mark.parametrize("exp,plan", [(6, 1), (8, 1), (9, 0), (10, 0), (10, 2), (11, 1), (8, 2), (11, 3),
+                                      (14, 2)])
 def test_plan_any_hop(runner, exp, plan):
-    """A plan other than 0 with an explicit geometry: hop 1, a hop past the
-    window (fewer frames per tile), against the FP64 pipeline."""
+    """A plan with an explicit geometry: hop 1, a hop past the window (fewer
+    frames per tile than the workgroup holds), both N/2+1 layouts, against
+    the FP64 pipeline (windows 512 and 1024 write these layouts a wave per
+    frame run)."""
     from test_mfma import _activate_geometry
 
     runner.engine_variant(exp, plan=plan)
@@ -181,10 +184,12 @@ def test_plan_any_hop(runner, exp, plan):
     for hop in (1, N_ // 2 + 3, 3 * N_ + 5):
         frames = 37
         x = rng.uniform(-1, 1, (frames - 1) * hop + N_ // 2).astype(np.float32)
-        p = ostft.make_params(exp, 0.5, N.WIN_HANNING, 0.0, N.OUT_POWER, N.FLAG_REMOVE_DC | N.FLAG_EXACT_WINDOW)
-        y = _activate_geometry(runner, x, p, hop, frames)
-        t = po.stft64(x, exp, hop, frames, po.WINDOWS["hanning"], 0.0, True, po.OUT_POWER, quirk_odd=False)
-        check(y, t, N_ // 2 + 1, f"exp {exp} hop {hop}")
+        for out, mode, per in ((N.OUT_POWER, po.OUT_POWER, N_ // 2 + 1),
+                               (N.OUT_HALF_SPECTRUM, po.OUT_HALF_SPECTRUM, N_ + 2)):
+            p = ostft.make_params(exp, 0.5, N.WIN_HANNING, 0.0, out, N.FLAG_REMOVE_DC | N.FLAG_EXACT_WINDOW)
+            y = _activate_geometry(runner, x, p, hop, frames)
+            t = po.stft64(x, exp, hop, frames, po.WINDOWS["hanning"], 0.0, True, mode, quirk_odd=False)
+            check(y, t, per, f"exp {exp} plan {plan} hop {hop} layout {out}")
 
 
 def test_autotune_picks_a_variant_and_keeps_results(runner):
