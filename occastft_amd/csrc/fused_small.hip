@@ -58,6 +58,11 @@ struct BatchWaves {
     static constexpr int value = (LOG_M == 9 || LOG_M == 11 || LOG_M >= 13) ? 1 : 4;
 };
 
+// Fold a last radix-2 Stockham pass into the epilogue (see fused_frames).
+#ifndef OSTFT_FOLD2
+#define OSTFT_FOLD2 1
+#endif
+
 // The per-frame slot epilogue of the N/2+1 layouts is used from
 // OSTFT_SLOT_MIN_T threads per transform (windows of 32 x that) on.
 #ifndef OSTFT_SLOT_MIN_T
@@ -251,6 +256,13 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
     // only.
     const int mode = a.out_mode;
     constexpr bool SYM_REAL = LOG_M >= OSTFT_SYM_LOG_M;
+    // A last radix-2 pass is folded into the epilogue (windows whose DC
+    // removal precedes the FFT): the epilogue reads Z[k] = X[k mod M/2] +
+    // W_M^k X[k mod M/2 + M/2] from the buffer the other passes leave, W_M^k
+    // being the square of the W_N^k it holds for the bin anyway.  That saves
+    // the pass's LDS round trip and barrier.
+    constexpr bool FOLD2 = OSTFT_FOLD2 && SYM_REAL && !DIRECT && LOG_M < OSTFT_FREQ_DC_LOG_M && C::NPASS >= 2 &&
+                           C::radix(C::NPASS - 1) == 2;
     constexpr int QPF = M / 8, QITER = (G * QPF) / WG;  // real layouts: quads
     constexpr int PPF = M / 4, PITER = (G * PPF) / WG;  // complex layout: pairs
     static_assert(QITER * WG == G * QPF && PITER * WG == G * PPF, "blocks divide evenly over the workgroup");
@@ -282,13 +294,14 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
 #pragma unroll
     for (int it = 0; it < NEB; ++it)
         eb[it] = ec1;
-    if (mode == OUT_HALF_COMPLEX && a.combine) {
+    const bool need_w = a.combine || FOLD2;  // the fold needs W_N^k at every bin
+    if (mode == OUT_HALF_COMPLEX && need_w) {
 #pragma unroll
         for (int it = 0; it < PITER; ++it)
             eb[it] = twn[2 * ((tid + it * WG) & (PPF - 1))];
         ec1 = twn[1];
         ec2 = twn[2];
-    } else if (vec_out && a.combine) {
+    } else if (vec_out && need_w) {
         if constexpr (SYM_REAL) {
 #pragma unroll
             for (int it = 0; it < QITER; ++it)
@@ -302,7 +315,7 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
         ec2 = twn[2];
         ec3 = twn[3];
         ec4 = twn[4];
-    } else if (a.combine) {
+    } else if (need_w) {
         if constexpr (SLOT_WAVE) {
             // the slot of lane l and step jj is the same in every frame
 #pragma unroll
@@ -422,7 +435,7 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
     // ---- 4. FFT of the packed frame; Z stays in LDS, natural order ----
     {
         first_pass_order<LOG_M, V>(v);
-        stockham<LOG_M, 0, V>(v, lds + grp * PADM, ptw, t);
+        stockham<LOG_M, 0, V, FOLD2 ? C::NPASS - 1 : C::NPASS>(v, lds + grp * PADM, ptw, t);
         if constexpr (FREQ_DC) {
             // z[n] - mu (1 + i) for all n, mu = (Re Z0 + Im Z0) / N, moves
             // Z0 to ((Re Z0 - Im Z0) / 2, (Im Z0 - Re Z0) / 2)
@@ -444,6 +457,17 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
     // kernel meets it at exponents 13 and 15 alone.
     float* __restrict__ out = a.out;
     constexpr bool VAR_COMBINE = LOG_M == 12 || LOG_M == 14;
+    // Z[k] of a frame's buffer z, k < M (k = M reads as 0); wm = W_M^k, used
+    // only when folding
+    auto zat = [&](const float2* z, uint32_t k, float2 wm) -> float2 {
+        if constexpr (FOLD2) {
+            const uint32_t k1 = k & (uint32_t)(M / 2 - 1);
+            return cadd(z[pad(k1)], cmul(wm, z[pad(k1 + M / 2)]));
+        } else {
+            (void)wm;
+            return z[pad(k & (uint32_t)(M - 1))];
+        }
+    };
     auto epilogue = [&](auto mode_c, auto comb_c) {
         constexpr int MODE = decltype(mode_c)::value;
         constexpr bool COMB = decltype(comb_c)::value;
@@ -467,9 +491,26 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
                 const float2* zf = z + pad(k0);
                 const float2* zm = z + pad((uint32_t)M - k0 - 2);
                 const float2 w1 = cmul(eb[it], ec1), w2 = cmul(eb[it], ec2);
-                const BinPair b0 = finish_pair<COMB>(zf[0], z[pad(((uint32_t)M - k0) & (M - 1))], eb[it]);
-                const BinPair b1 = finish_pair<COMB>(zf[1], zm[1], w1);
-                const BinPair b2 = finish_pair<COMB>(z[pad(k0 + 2)], zm[0], w2);
+                float2 zk[3], zr[3];  // Z[k0 + j], Z[M - k0 - j]
+                if constexpr (FOLD2) {
+                    const float2 ws[3] = {eb[it], w1, w2};
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) {
+                        const float2 wm = cmul(ws[j], ws[j]);
+                        zk[j] = zat(z, k0 + j, wm);
+                        zr[j] = zat(z, (uint32_t)M - k0 - j, make_float2(wm.x, -wm.y));
+                    }
+                } else {
+                    zk[0] = zf[0];
+                    zk[1] = zf[1];
+                    zk[2] = z[pad(k0 + 2)];
+                    zr[0] = z[pad(((uint32_t)M - k0) & (M - 1))];
+                    zr[1] = zm[1];
+                    zr[2] = zm[0];
+                }
+                const BinPair b0 = finish_pair<COMB>(zk[0], zr[0], eb[it]);
+                const BinPair b1 = finish_pair<COMB>(zk[1], zr[1], w1);
+                const BinPair b2 = finish_pair<COMB>(zk[2], zr[2], w2);
                 const uint64_t base = (f0 + g) * (uint64_t)N;
                 const uint64_t c0 = base + k0, c1 = base + M + k0, c2 = base + hm - k0 - 2, c3 = base + lm - k0 - 2;
                 const float2 v0 = value_of(b0.lo), v1 = value_of(b1.lo);
@@ -507,11 +548,20 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
                     const float2 wk[5] = {eb[it], cmul(eb[it], ec1), cmul(eb[it], ec2), cmul(eb[it], ec3),
                                           cmul(eb[it], ec4)};
                     BinPair b[5];
-                    b[0] = finish_pair<COMB>(zf[0], z[pad(((uint32_t)M - k0) & (M - 1))], wk[0]);
+                    if constexpr (FOLD2) {
 #pragma unroll
-                    for (int j = 1; j < 4; ++j)
-                        b[j] = finish_pair<COMB>(zf[j], zm[4 - j], wk[j]);
-                    b[4] = finish_pair<COMB>(z[pad(k0 + 4)], zm[0], wk[4]);
+                        for (int j = 0; j < 5; ++j) {
+                            const float2 wm = cmul(wk[j], wk[j]);
+                            b[j] = finish_pair<COMB>(zat(z, k0 + j, wm),
+                                                     zat(z, (uint32_t)M - k0 - j, make_float2(wm.x, -wm.y)), wk[j]);
+                        }
+                    } else {
+                        b[0] = finish_pair<COMB>(zf[0], z[pad(((uint32_t)M - k0) & (M - 1))], wk[0]);
+#pragma unroll
+                        for (int j = 1; j < 4; ++j)
+                            b[j] = finish_pair<COMB>(zf[j], zm[4 - j], wk[j]);
+                        b[4] = finish_pair<COMB>(z[pad(k0 + 4)], zm[0], wk[4]);
+                    }
                     // the value of a conjugate is the value itself
                     float lo[5], hi[5];
 #pragma unroll
@@ -567,8 +617,16 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
 #pragma unroll
                     for (uint32_t jj = 0; jj < JW; ++jj) {
                         const uint32_t k = lane + 64 * jj;
-                        const float2 zm = (jj == 0 && lane == 0) ? z[0] : zh[68 * (JW - 1 - jj)];
-                        const BinPair b = finish_pair<COMB>(zl[68 * jj], zm, eb[jj]);
+                        float2 zk, zm;
+                        if constexpr (FOLD2) {
+                            const float2 wm = cmul(eb[jj], eb[jj]);
+                            zk = zat(z, k, wm);
+                            zm = zat(z, (uint32_t)M - k, make_float2(wm.x, -wm.y));
+                        } else {
+                            zk = zl[68 * jj];
+                            zm = (jj == 0 && lane == 0) ? z[0] : zh[68 * (JW - 1 - jj)];
+                        }
+                        const BinPair b = finish_pair<COMB>(zk, zm, eb[jj]);
                         const float2 x2 = (COMB || k == 0) ? b.hi : b.lo;  // position M-k, before conj
                         if constexpr (MODE == OUT_POWER) {
                             half_store(power_of(b.lo), out + o + lane + 64 * jj);
@@ -585,7 +643,8 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
             const uint32_t gm = wv * FW + lane;
             if (lane < FW && gm < nf) {
                 const float2* z = lds + gm * PADM;
-                const BinPair b = finish_pair<COMB>(z[pad(M / 2)], z[pad(M / 2)], eb[JW]);
+                const float2 zh2 = zat(z, M / 2, cmul(eb[JW], eb[JW]));  // W_M^(M/2) = -1 when folding
+                const BinPair b = finish_pair<COMB>(zh2, zh2, eb[JW]);
                 const uint64_t o = (f0 + gm) * (uint64_t)(M + 1) + M / 2;
                 if constexpr (MODE == OUT_POWER)
                     half_store(power_of(b.lo), out + o);
@@ -616,8 +675,16 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
 #pragma unroll
                 for (int j = 0; j < VPT / 2; ++j) {
                     const uint32_t k = t + j * T;
-                    const float2 zm = (j == 0 && t == 0) ? z[0] : zh[(VPT / 2 - 1 - j) * PT];
-                    const BinPair b = finish_pair<COMB>(zl[j * PT], zm, eb[j]);
+                    float2 zk, zm;
+                    if constexpr (FOLD2) {
+                        const float2 wm = cmul(eb[j], eb[j]);
+                        zk = zat(z, k, wm);
+                        zm = zat(z, (uint32_t)M - k, make_float2(wm.x, -wm.y));
+                    } else {
+                        zk = zl[j * PT];
+                        zm = (j == 0 && t == 0) ? z[0] : zh[(VPT / 2 - 1 - j) * PT];
+                    }
+                    const BinPair b = finish_pair<COMB>(zk, zm, eb[j]);
                     const float2 x2 = (COMB || k == 0) ? b.hi : b.lo;  // position M-k, before conj
                     if constexpr (MODE == OUT_POWER) {
                         float* ol = out + o + t;
@@ -632,7 +699,8 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
                     }
                 }
                 if (t == 0) {  // slot M/2: its own mirror
-                    const BinPair b = finish_pair<COMB>(z[pad(M / 2)], z[pad(M / 2)], eb[VPT / 2]);
+                    const float2 zh2 = zat(z, M / 2, cmul(eb[VPT / 2], eb[VPT / 2]));
+                    const BinPair b = finish_pair<COMB>(zh2, zh2, eb[VPT / 2]);
                     if constexpr (MODE == OUT_POWER)
                         half_store(power_of(b.lo), out + o + M / 2);
                     else
@@ -647,6 +715,7 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
             // X[M] or O[0].  Consecutive slots sit on consecutive lanes, so each
             // store instruction covers contiguous memory (ascending for k,
             // descending for M-k).
+            static_assert(!FOLD2, "no built variant folds here");
             constexpr uint32_t SPF = M / 2 + 1;  // slots per frame
             const uint32_t nslot = nf * SPF;
             const uint64_t obase = f0 * (uint64_t)(M + 1);

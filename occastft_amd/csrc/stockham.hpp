@@ -193,9 +193,10 @@ __device__ __forceinline__ void first_pass_order(float2 (&v)[16])
 }
 
 // Stockham pass PASS: twiddle, radix-r DFTs, scatter to LDS; then gather the
-// inputs of the next pass (or stop with the spectrum in LDS).  No global
-// memory is touched here.
-template <int LOG_M, int PASS, int V = 0>
+// inputs of the next pass, or stop after pass NRUN - 1 with that pass's
+// output in LDS (the spectrum when NRUN is every pass).  No global memory is
+// touched here.
+template <int LOG_M, int PASS, int V = 0, int NRUN = Cfg<LOG_M, V>::NPASS>
 __device__ __forceinline__ void stockham(float2 (&v)[16], float2* z, const PassTwiddles<LOG_M, V>& P, uint32_t t)
 {
     using C = Cfg<LOG_M, V>;
@@ -250,7 +251,7 @@ __device__ __forceinline__ void stockham(float2 (&v)[16], float2* z, const PassT
         });
     });
     __syncthreads();
-    if constexpr (PASS + 1 < C::NPASS) {
+    if constexpr (PASS + 1 < NRUN) {
         constexpr int r2 = C::radix(PASS + 1), U2 = C::VPT / r2;
         cfor<0, U2>([&](auto u) {
             constexpr int UU = decltype(u)::value;
@@ -261,7 +262,7 @@ __device__ __forceinline__ void stockham(float2 (&v)[16], float2* z, const PassT
             });
         });
         __syncthreads();
-        stockham<LOG_M, PASS + 1, V>(v, z, P, t);
+        stockham<LOG_M, PASS + 1, V, NRUN>(v, z, P, t);
     }
 }
 
