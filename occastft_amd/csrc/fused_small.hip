@@ -261,7 +261,7 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
     // W_M^k X[k mod M/2 + M/2] from the buffer the other passes leave, W_M^k
     // being the square of the W_N^k it holds for the bin anyway.  That saves
     // the pass's LDS round trip and barrier.
-    constexpr bool FOLD2 = OSTFT_FOLD2 && SYM_REAL && !DIRECT && LOG_M < OSTFT_FREQ_DC_LOG_M && C::NPASS >= 2 &&
+    constexpr bool FOLD2 = OSTFT_FOLD2 && !DIRECT && LOG_M < OSTFT_FREQ_DC_LOG_M && C::NPASS >= 2 &&
                            C::radix(C::NPASS - 1) == 2;
     constexpr int QPF = M / 8, QITER = (G * QPF) / WG;  // real layouts: quads
     constexpr int PPF = M / 4, PITER = (G * PPF) / WG;  // complex layout: pairs
@@ -589,7 +589,9 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
 #pragma unroll
                     for (uint32_t j = 0; j < 4; ++j) {
                         const uint32_t k = k0 + j, km = (M - k) & (M - 1);
-                        const BinPair b = finish_pair<COMB>(z[pad(k)], z[pad(km)], wk[j]);
+                        const float2 wm = cmul(wk[j], wk[j]);  // W_M^k (the fold only)
+                        const BinPair b = finish_pair<COMB>(zat(z, k, wm), zat(z, km, make_float2(wm.x, -wm.y)),
+                                                            wk[j]);
                         lo[j] = conv(b.lo);
                         hi[j] = conv(b.hi);
                     }
@@ -761,7 +763,9 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
                 const uint32_t kb1 = k1 == (uint32_t)M ? 0u : k1;
                 const float2 w1 = eb[it];
                 const float2* z1 = lds + g1 * PADM;
-                const BinPair b1 = finish_pair<COMB>(z1[pad(kb1)], z1[pad((M - kb1) & (M - 1))], w1);
+                const float2 wm1 = cmul(w1, w1);  // W_M^kb1 (the fold only)
+                const BinPair b1 = finish_pair<COMB>(zat(z1, kb1, wm1),
+                                                     zat(z1, (M - kb1) & (M - 1), make_float2(wm1.x, -wm1.y)), w1);
                 const float2 x1 = k1 == (uint32_t)M ? b1.hi : b1.lo;  // doubled
                 // the next value: bin k1 + 1 of the same frame, or bin 0 of the next
                 const uint32_t g2 = k1 == (uint32_t)M ? g1 + 1 : g1, k2 = k1 == (uint32_t)M ? 0u : k1 + 1;
@@ -770,7 +774,9 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
                 if (has2) {
                     const float2 w2 = kb2 != 0u ? cmul(w1, ec1) : one;
                     const float2* z2 = lds + g2 * PADM;
-                    const BinPair b2 = finish_pair<COMB>(z2[pad(kb2)], z2[pad((M - kb2) & (M - 1))], w2);
+                    const float2 wm2 = cmul(w2, w2);
+                    const BinPair b2 = finish_pair<COMB>(zat(z2, kb2, wm2),
+                                                         zat(z2, (M - kb2) & (M - 1), make_float2(wm2.x, -wm2.y)), w2);
                     x2 = k2 == (uint32_t)M ? b2.hi : b2.lo;
                 }
                 const uint64_t o = obase + e1;
