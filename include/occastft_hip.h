@@ -247,11 +247,17 @@ int ostft_engine_variant(ostft_ctx* ctx, int32_t exp, int32_t shape, int32_t pla
  * and launch form (one workgroup per tile; persistent at every
  * workgroups-per-CU setting, shape 0), keeps the fastest for this context
  * and reports it (*best_blocks_per_cu = 0 for one workgroup per tile; any
- * output pointer may be NULL).  The plan it keeps applies to every FP32
- * request of that window on the context (see ostft_engine_variant); while it
- * runs, other calls on the context keep the previous plan. */
-int ostft_autotune_batch(ostft_ctx* ctx, int32_t exp, uint64_t samples, int32_t* best_blocks_per_cu,
-                         int32_t* best_shape, int32_t* best_plan, double* best_ms);
+ * output pointer may be NULL).  The synthetic requests take the overlap,
+ * window, output layout and flags of `like` (its exp is ignored; a
+ * matrix-core flag is OSTFT_ERR_INVALID_ARG), or with `like` NULL overlap
+ * 0.5, Hann, DC removal and the reference power layout: variants rank
+ * differently under different layouts, so tune for the requests the
+ * context will serve.  The default variant stays unless a candidate is more
+ * than 2% faster.  The plan it keeps applies to every FP32 request of that
+ * window on the context (see ostft_engine_variant); while it runs, other
+ * calls on the context keep the previous plan. */
+int ostft_autotune_batch(ostft_ctx* ctx, int32_t exp, uint64_t samples, const ostft_params* like,
+                         int32_t* best_blocks_per_cu, int32_t* best_shape, int32_t* best_plan, double* best_ms);
 
 /* Scratch budget of the two-pass path (windows 2^16..2^20), in bytes: its
  * frames go through both passes in chunks whose intermediate spectra fit the
@@ -281,6 +287,9 @@ typedef struct ostft_server_config {
     uint32_t max_inflight;  /* batches on the GPU at once; 0: 3 */
     int32_t use_graphs;     /* 1 (default when the config is NULL): replay captured launches */
     int32_t autotune;       /* 1: autotune every batch window size (ostft_autotune_batch) at creation */
+    const ostft_params* autotune_like; /* the requests to tune for (NULL: ostft_autotune_batch's default);
+                                          read at creation only */
+    uint64_t autotune_samples;  /* samples of each tuning batch (32 requests); 0: 2^26 */
 } ostft_server_config;
 
 typedef struct ostft_server_stats {

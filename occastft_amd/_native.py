@@ -122,6 +122,8 @@ class ServerConfig(ctypes.Structure):
         ("max_inflight", ctypes.c_uint32),
         ("use_graphs", ctypes.c_int32),
         ("autotune", ctypes.c_int32),
+        ("autotune_like", ctypes.c_void_p),
+        ("autotune_samples", ctypes.c_uint64),
     ]
 
 
@@ -198,8 +200,8 @@ def lib() -> ctypes.CDLL:
     L.ostft_batch_config.restype = ctypes.c_int
     L.ostft_batch_config.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, i32p]
     L.ostft_autotune_batch.restype = ctypes.c_int
-    L.ostft_autotune_batch.argtypes = [vp, ctypes.c_int32, ctypes.c_uint64, i32p, i32p, i32p,
-                                       ctypes.POINTER(ctypes.c_double)]
+    L.ostft_autotune_batch.argtypes = [vp, ctypes.c_int32, ctypes.c_uint64, ctypes.POINTER(Params), i32p, i32p,
+                                       i32p, ctypes.POINTER(ctypes.c_double)]
     L.ostft_engine_variant.restype = ctypes.c_int
     L.ostft_engine_variant.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, i32p, i32p]
     L.ostft_scratch_budget.restype = ctypes.c_int

@@ -520,15 +520,18 @@ int engine_variant(ostft_ctx* ctx, int exp, int shape, int plan, int* cur_shape,
 }
 
 // Times one class over a synthetic ragged batch (`samples` samples in 32
-// requests, overlap 0.5, Hann, DC removal, compat power) under every built
+// requests with the window, overlap, output layout and flags of `like`, or
+// without it overlap 0.5, Hann, DC removal, compat power) under every built
 // variant (radix plan x workgroup shape) in the dispatch form, and for
 // shape 0 in the persistent form at each workgroups-per-CU setting.  The
 // default (variant 0, one workgroup per tile) is timed first and stays unless
 // a candidate beats it by more than AUTOTUNE_MARGIN: most candidates land
 // within a percent or two of each other, where one timing's noise would
-// otherwise decide.
-int autotune_batch(ostft_ctx* ctx, int exp, uint64_t samples, int* best_blocks_per_cu, int* best_shape,
-                   int* best_plan, double* best_ms)
+// otherwise decide.  Which variant wins depends on the requests: the N/2+1
+// layouts write half the bytes of the N-value ones per frame, so the same
+// candidates rank differently under them.
+int autotune_batch(ostft_ctx* ctx, int exp, uint64_t samples, const ostft_params* like, int* best_blocks_per_cu,
+                   int* best_shape, int* best_plan, double* best_ms)
 {
     BatchState* st = nullptr;
     int rc = state(ctx, &st);
@@ -566,12 +569,20 @@ int autotune_batch(ostft_ctx* ctx, int exp, uint64_t samples, int* best_blocks_p
     for (int i = 0; i < 32; ++i)
         lens.push_back(per / 2 + (per * (uint64_t)(i % 8)) / 8 + (uint64_t)(i * 37) % N);
     ostft_params p;
-    std::memset(&p, 0, sizeof(p));
-    p.exp = exp;
-    p.overlap = 0.5f;
-    p.window = OSTFT_WIN_HANNING;
-    p.output = OSTFT_OUT_COMPAT_POWER;
-    p.flags = OSTFT_FLAG_REMOVE_DC;
+    if (like) {
+        p = *like;
+        p.exp = exp;
+        if (p.flags & (OSTFT_FLAG_MFMA_FP16 | OSTFT_FLAG_MFMA_BF16))
+            return ctx_fail(ctx, OSTFT_ERR_INVALID_ARG,
+                            "autotune: matrix-core requests do not run the batch kernels");
+    } else {
+        std::memset(&p, 0, sizeof(p));
+        p.exp = exp;
+        p.overlap = 0.5f;
+        p.window = OSTFT_WIN_HANNING;
+        p.output = OSTFT_OUT_COMPAT_POWER;
+        p.flags = OSTFT_FLAG_REMOVE_DC;
+    }
     std::vector<ostft_job> jobs(lens.size());
     uint64_t in_off = 0, out_off = 0;
     for (size_t i = 0; i < lens.size(); ++i) {

@@ -27,8 +27,8 @@ int ctx_prepare(ostft_ctx* ctx, const ostft_params* params, const float* d_in, u
 int run_batch(ostft_ctx* ctx, const ostft_job* jobs, uint32_t njobs, hipStream_t s);
 int batch_config(ostft_ctx* ctx, int exp, int blocks_per_cu, int use_graphs, int* current);
 int engine_variant(ostft_ctx* ctx, int exp, int shape, int plan, int* cur_shape, int* cur_plan);
-int autotune_batch(ostft_ctx* ctx, int exp, uint64_t samples, int* best_blocks_per_cu, int* best_shape,
-                   int* best_plan, double* best_ms);
+int autotune_batch(ostft_ctx* ctx, int exp, uint64_t samples, const ostft_params* like, int* best_blocks_per_cu,
+                   int* best_shape, int* best_plan, double* best_ms);
 void batch_release(ostft_ctx* ctx);
 
 }  // namespace ostft

@@ -767,8 +767,8 @@ int ostft_engine_variant(ostft_ctx* ctx, int32_t exp, int32_t shape, int32_t pla
     return ostft::engine_variant(ctx, exp, shape, plan, cur_shape, cur_plan);
 }
 
-int ostft_autotune_batch(ostft_ctx* ctx, int32_t exp, uint64_t samples, int32_t* best_blocks_per_cu,
-                         int32_t* best_shape, int32_t* best_plan, double* best_ms)
+int ostft_autotune_batch(ostft_ctx* ctx, int32_t exp, uint64_t samples, const ostft_params* like,
+                         int32_t* best_blocks_per_cu, int32_t* best_shape, int32_t* best_plan, double* best_ms)
 {
     if (!ctx)
         return OSTFT_ERR_INVALID_ARG;
@@ -776,7 +776,7 @@ int ostft_autotune_batch(ostft_ctx* ctx, int32_t exp, uint64_t samples, int32_t*
     if (!dg.ok)
         return fail(ctx, OSTFT_ERR_NO_DEVICE, "hipSetDevice failed");
     std::lock_guard<std::mutex> lk(ctx->batch_mu);
-    return ostft::autotune_batch(ctx, exp, samples, best_blocks_per_cu, best_shape, best_plan, best_ms);
+    return ostft::autotune_batch(ctx, exp, samples, like, best_blocks_per_cu, best_shape, best_plan, best_ms);
 }
 
 int ostft_scratch_budget(ostft_ctx* ctx, uint64_t bytes, uint64_t* current)

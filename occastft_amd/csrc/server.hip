@@ -426,6 +426,7 @@ int ostft_server_create(int32_t device, const ostft_server_config* config, ostft
     std::unique_ptr<ostft_server> s(new ostft_server());
     s->device = device;
     s->cfg = cfg;
+    s->cfg.autotune_like = nullptr;  // read at creation only
     int rc = ostft_create(device, &s->ctx);
     if (rc != OSTFT_OK)
         return rc;
@@ -452,7 +453,8 @@ int ostft_server_create(int32_t device, const ostft_server_config* config, ostft
         rc = ostft_batch_config(s->ctx, 0, 0, cfg.use_graphs ? 1 : 0, nullptr);
     if (e == hipSuccess && rc == OSTFT_OK && cfg.autotune) {
         for (int exp = 6; exp <= 15 && rc == OSTFT_OK; ++exp)
-            rc = ostft_autotune_batch(s->ctx, exp, 1ull << 26, nullptr, nullptr, nullptr, nullptr);
+            rc = ostft_autotune_batch(s->ctx, exp, cfg.autotune_samples ? cfg.autotune_samples : 1ull << 26,
+                                      cfg.autotune_like, nullptr, nullptr, nullptr, nullptr);
     }
     if (e != hipSuccess || rc != OSTFT_OK) {
         if (s->arena_d)

@@ -227,17 +227,21 @@ class Runner:
             raise STFTError(rc, self.last_error)
         return cs.value, cp.value
 
-    def autotune_batch(self, exp: int, samples: int = 1 << 24) -> "AutotuneChoice":
+    def autotune_batch(self, exp: int, samples: int = 1 << 24, like: N.Params | None = None) -> "AutotuneChoice":
         """Times every built variant and launch form of window exponent `exp`
         on a synthetic ragged batch, keeps the fastest for this context and
         returns it: blocks_per_cu (0 = one workgroup per tile, k > 0 =
-        persistent with k workgroups per CU), shape, plan, ms."""
+        persistent with k workgroups per CU), shape, plan, ms.  The synthetic
+        requests take the overlap, window, output layout and flags of `like`
+        (a Params, e.g. from parse_options; its exp is ignored), or without
+        it overlap 0.5, Hann, DC removal and the reference power layout."""
         best = ctypes.c_int32()
         shape = ctypes.c_int32()
         plan = ctypes.c_int32()
         ms = ctypes.c_double()
-        rc = N.lib().ostft_autotune_batch(self._h, int(exp), int(samples), ctypes.byref(best), ctypes.byref(shape),
-                                          ctypes.byref(plan), ctypes.byref(ms))
+        rc = N.lib().ostft_autotune_batch(self._h, int(exp), int(samples),
+                                          ctypes.byref(like) if like is not None else None, ctypes.byref(best),
+                                          ctypes.byref(shape), ctypes.byref(plan), ctypes.byref(ms))
         if rc != N.OK:
             raise STFTError(rc, self.last_error)
         return AutotuneChoice(best.value, shape.value, plan.value, ms.value)
@@ -259,9 +263,11 @@ class Server:
     request is waited for."""
 
     def __init__(self, device: int = 0, arena_bytes: int = 0, max_batch: int = 0, max_inflight: int = 0,
-                 use_graphs: bool = True, autotune: bool = False):
+                 use_graphs: bool = True, autotune: bool = False, autotune_like: N.Params | None = None,
+                 autotune_samples: int = 0):
+        like = ctypes.cast(ctypes.pointer(autotune_like), ctypes.c_void_p) if autotune_like is not None else None
         cfg = N.ServerConfig(int(arena_bytes), int(max_batch), int(max_inflight), 1 if use_graphs else 0,
-                             1 if autotune else 0)
+                             1 if autotune else 0, like, int(autotune_samples))
         h = ctypes.c_void_p()
         rc = N.lib().ostft_server_create(int(device), ctypes.byref(cfg), ctypes.byref(h))
         if rc != N.OK:

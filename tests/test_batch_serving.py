@@ -233,6 +233,27 @@ def test_server_results_alone_and_in_the_mix(runner):
     assert 0 < st["latency_p50_ms"] <= st["latency_p99_ms"] <= st["latency_max_ms"]
 
 
+def test_server_autotuned_for_a_template(runner):
+    """A server autotuned at creation for a template request (its context
+    keeps whatever variants the tuner picked, so results need not be bitwise
+    those of another context) returns every request within the parity bar
+    of the FP64 pipeline."""
+    opts = "--hanning_window --remove_dc --exact_window --exact_odd_exp --power_return"
+    like = ostft.parse_options(opts, 10, 0.75)
+    rng = np.random.default_rng(77)
+    with ostft.Server(0, arena_bytes=64 << 20, autotune=True, autotune_like=like, autotune_samples=1 << 20) as srv:
+        for e in (8, 11, 13):
+            n = 1 << e
+            x = rng.uniform(-1, 1, 7 * n + 123).astype(np.float32)
+            p = ostft.parse_options(opts, e, 0.75)
+            y = srv.wait(srv.submit(x, p))
+            g = ostft.geometry(p, len(x))
+            t = po.stft64(x, e, g["hop"], g["frames"], po.WINDOWS["hanning"], 0.0, True, po.OUT_POWER,
+                          quirk_odd=False)
+            mx, p50, zabs = frame_errors(y, t, n // 2 + 1)
+            assert mx <= 1e-5 and zabs <= 1e-6, f"exp {e}: max per-frame rel err {mx:.3e}"
+
+
 def test_server_full_request_distribution(runner):
     """200 requests of the serving workload's distribution (lengths
     log-uniform 2^14..2^22, windows 2^8..2^13, overlap 0/25/50/75%, Hann /

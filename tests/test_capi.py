@@ -155,7 +155,7 @@ def test_context_calls_reject_a_null_context():
     assert N.lib().ostft_engine_variant(None, 10, 0, 0, ctypes.byref(shape), ctypes.byref(plan)) == N.ERR_INVALID_ARG
     assert shape.value == -5 and plan.value == -5
     ms = ctypes.c_double(-1.0)
-    assert N.lib().ostft_autotune_batch(None, 10, 1 << 20, ctypes.byref(best), ctypes.byref(shape),
+    assert N.lib().ostft_autotune_batch(None, 10, 1 << 20, None, ctypes.byref(best), ctypes.byref(shape),
                                         ctypes.byref(plan), ctypes.byref(ms)) == N.ERR_INVALID_ARG
     assert ms.value == -1.0
 

@@ -192,13 +192,18 @@ def test_plan_any_hop(runner, exp, plan):
             check(y, t, per, f"exp {exp} plan {plan} hop {hop} layout {out}")
 
 
-def test_autotune_picks_a_variant_and_keeps_results(runner):
+@pytest.mark.parametrize("like", [None, "--hamming_window --power_return@0.75", "--half_spectrum_return@0"])
+def test_autotune_picks_a_variant_and_keeps_results(runner, like):
     import torch
 
     from test_batch_serving import device_inputs, ragged_requests, run_alone
 
     for e in (7, 10, 14):
-        choice = runner.autotune_batch(e, 1 << 23)
+        lp = None
+        if like:
+            opts, ov = like.split("@")
+            lp = ostft.parse_options(opts, 11, float(ov))  # its exp is ignored
+        choice = runner.autotune_batch(e, 1 << 23, like=lp)
         assert choice.ms > 0 and choice.blocks_per_cu >= 0
         assert runner.engine_variant(e) == (choice.shape, choice.plan)
         assert runner.batch_config(e) == choice.blocks_per_cu
@@ -211,3 +216,14 @@ def test_autotune_picks_a_variant_and_keeps_results(runner):
         torch.cuda.synchronize()
         for o, a in zip(outs, alone):
             assert bitwise_equal(o.cpu().numpy(), a)
+
+
+def test_autotune_rejects_a_bad_like(runner):
+    """A template the batch kernels cannot run, or an invalid one, is refused
+    before anything is timed, and the context's settings stay."""
+    before = runner.engine_variant(10), runner.batch_config(10)
+    for bad in (ostft.parse_options("--hanning_window --mfma_fp16", 10, 0.5), ostft.make_params(10, 1.0)):
+        with pytest.raises(ostft.STFTError) as ei:
+            runner.autotune_batch(10, 1 << 20, like=bad)
+        assert ei.value.status == N.ERR_INVALID_ARG
+    assert (runner.engine_variant(10), runner.batch_config(10)) == before
