@@ -261,3 +261,29 @@ def test_matrix_core_form_empty_signal(runner):
         y = runner.run_device(torch.zeros(0, dtype=torch.float32, device="cuda"), p)
         torch.cuda.synchronize()
         assert y.numel() == g["out_floats"]
+
+
+@pytest.mark.parametrize("exp", [8])
+@pytest.mark.parametrize("flags", [N.FLAG_MFMA_FP16, N.FLAG_MFMA_BF16])
+def test_vector_and_scalar_loads_agree_bitwise(runner, exp, flags):
+    """Window 256: the 16-byte-load path (input, hop and stride multiples of
+    4 floats, each quad's blocks transposed across its lanes) and the
+    per-sample path (the same signal 4 bytes off a 16-byte boundary) hand
+    every lane the same samples, so results are bitwise equal."""
+    import torch
+
+    Nw = 1 << exp
+    rng = np.random.default_rng(900 + exp)
+    L = 300 * Nw + 11
+    x = torch.from_numpy(rng.uniform(-1, 1, L).astype(np.float32)).cuda()
+    buf = torch.zeros(L + 4, dtype=torch.float32, device="cuda")
+    buf[1: 1 + L] = x
+    xm = buf[1: 1 + L]
+    assert x.data_ptr() % 16 == 0 and xm.data_ptr() % 16 == 4
+    for out_mode in (N.OUT_POWER, N.OUT_HALF_COMPLEX, N.OUT_COMPAT_POWER):
+        p = ostft.make_params(exp, 0.5, N.WIN_HANNING, 0.0, out_mode, N.FLAG_REMOVE_DC | flags)
+        a = runner.run_device(x, p)
+        b = runner.run_device(xm, p)
+        torch.cuda.synchronize()
+        a, b = a.cpu().numpy(), b.cpu().numpy()
+        assert a.shape == b.shape and np.array_equal(a.view(np.uint32), b.view(np.uint32)), (exp, out_mode)
