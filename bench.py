@@ -381,14 +381,19 @@ def bench_sweep(args, runner):
     for _ in range(args.warmup):
         run_all()
     torch.cuda.synchronize()
+    # the timed steps carry no per-request events (each would add a command
+    # between the launches); the per-request times come from as many steps
+    # run again with events
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        run_all()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
     ev = [[(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in reqs]
           for _ in range(args.steps)]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
     for s in range(args.steps):
         run_all(ev[s])
     torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
     per_ms = np.array([[ev[s][i][0].elapsed_time(ev[s][i][1]) for i in range(len(reqs))]
                        for s in range(args.steps)]).mean(axis=0)
     frames = sum(g["compute_frames"] for *_, g in reqs)
