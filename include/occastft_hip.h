@@ -253,7 +253,8 @@ int ostft_engine_variant(ostft_ctx* ctx, int32_t exp, int32_t shape, int32_t pla
  * 0.5, Hann, DC removal and the reference power layout: variants rank
  * differently under different layouts, so tune for the requests the
  * context will serve.  The default variant stays unless a candidate is more
- * than 2% faster.  The plan it keeps applies to every FP32 request of that
+ * than 2% faster, in the sweep and again when the two are timed
+ * alternately.  The plan it keeps applies to every FP32 request of that
  * window on the context (see ostft_engine_variant); while it runs, other
  * calls on the context keep the previous plan. */
 int ostft_autotune_batch(ostft_ctx* ctx, int32_t exp, uint64_t samples, const ostft_params* like,
