@@ -64,9 +64,10 @@ struct BatchWaves {
 #endif
 
 // The per-frame slot epilogue of the N/2+1 layouts is used from
-// OSTFT_SLOT_MIN_T threads per transform (windows of 32 x that) on.
+// OSTFT_SLOT_MIN_T threads per transform (windows of 32 x that with 16 values
+// per thread) on.
 #ifndef OSTFT_SLOT_MIN_T
-#define OSTFT_SLOT_MIN_T 16
+#define OSTFT_SLOT_MIN_T 8
 #endif
 // Stores of the N/2+1 layouts (POWER, HALF_SPECTRUM): plain by default.  A
 // frame's M+1 values start at any 4- or 8-byte boundary, so a wave's store
@@ -271,7 +272,7 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
     // and iteration; at the smallest window two consecutive values of the
     // flattened [frame][bin] index)
     constexpr int OITERS = (G * (M / 2 + 1) + WG - 1) / WG;
-    // From T >= 16 (windows from 512) the N/2+1 layouts are written frame by
+    // From T >= 8 (windows from 256) the N/2+1 layouts are written frame by
     // frame: slots k < M/2 of a frame as k = t + j T (j < VPT/2) by its own group
     // when a group spans whole waves (T >= 64), else as k = lane + 64 jj
     // (jj < VPT T/128) by a whole wave, frame after frame; the middle slot k = M/2
@@ -279,6 +280,7 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
     // plus a constant, and each store instruction one contiguous run.
     constexpr bool SLOT_FRAME = SYM_REAL && T >= OSTFT_SLOT_MIN_T;
     constexpr bool SLOT_WAVE = SLOT_FRAME && T < 64;
+    static_assert(!SLOT_WAVE || VPT * T >= 128, "a wave's 64 lanes take slots k < M/2 in whole runs");
     // at the largest transforms (one frame per workgroup, four passes) the
     // slot twiddles are loaded after the FFT, which keeps their registers free
     // through it
