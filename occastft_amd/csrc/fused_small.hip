@@ -58,6 +58,13 @@ struct BatchWaves {
     static constexpr int value = (LOG_M == 9 || LOG_M == 11 || LOG_M >= 13) ? 1 : 4;
 };
 
+// The N/2+1 layouts of the smallest window in per-frame slots (fused_frames):
+// one pair of bins per slot and no division per value, where the flattened
+// form evaluated a pair per value.
+#ifndef OSTFT_SLOT_SMALL
+#define OSTFT_SLOT_SMALL 1
+#endif
+
 // Fold a last radix-2 Stockham pass into the epilogue (see fused_frames).
 #ifndef OSTFT_FOLD2
 #define OSTFT_FOLD2 1
@@ -281,6 +288,10 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
     constexpr bool SLOT_FRAME = SYM_REAL && T >= OSTFT_SLOT_MIN_T;
     constexpr bool SLOT_WAVE = SLOT_FRAME && T < 64;
     static_assert(!SLOT_WAVE || VPT * T >= 128, "a wave's 64 lanes take slots k < M/2 in whole runs");
+    // Below the symmetry threshold (the smallest window) the same slots, with
+    // 64 / (M/2) frames side by side in a wave (lane = frame x M/2 + slot)
+    constexpr bool SLOT_SMALL = OSTFT_SLOT_SMALL && !SYM_REAL && !DIRECT && M / 2 <= 64 && 64 % (M / 2) == 0 &&
+                                (64 * VPT / M) % (128 / M) == 0;
     // at the largest transforms (one frame per workgroup, four passes) the
     // slot twiddles are loaded after the FFT, which keeps their registers free
     // through it
@@ -338,6 +349,10 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
                 if (e < (uint32_t)(G * (M / 2 + 1)))
                     eb[it] = twn[e % (uint32_t)(M / 2 + 1)];
             }
+        } else if constexpr (SLOT_SMALL) {
+            // the slot of a lane is the same in every frame it takes
+            eb[0] = twn[(tid & 63u) & (uint32_t)(M / 2 - 1)];
+            eb[1] = twn[M / 2];
         } else {
 #pragma unroll
             for (int it = 0; it < OITERP; ++it) {
@@ -710,6 +725,47 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
                     else
                         half_store(value_of(b.lo), reinterpret_cast<float2*>(out) + o + M / 2);
                 }
+            }
+        } else if constexpr (SLOT_SMALL) {
+            // POWER / HALF_SPECTRUM at the smallest window, M+1 values per
+            // frame: a wave's FW frames, FS at a time, lane = frame within the
+            // step x M/2 + slot k (slots as in the SLOT_WAVE branch); the
+            // middle slots apart, one per lane
+            constexpr uint32_t S = M / 2, FS = 64 / S, FW = 64 * VPT / M;
+            const uint32_t lane = tid & 63u, wv = tid >> 6;
+            const uint32_t k = lane & (S - 1u), fs = lane / S;
+            const float2 wm = cmul(eb[0], eb[0]);  // W_M^k (the fold only)
+#pragma unroll
+            for (uint32_t st = 0; st < FW / FS; ++st) {
+                const uint32_t g = wv * FW + st * FS + fs;
+                if (g < nf) {
+                    const float2* z = lds + g * PADM;
+                    const float2 zk = zat(z, k, wm);
+                    const float2 zm = FOLD2 ? zat(z, (uint32_t)M - k, make_float2(wm.x, -wm.y))
+                                            : z[pad(((uint32_t)M - k) & (M - 1))];
+                    const BinPair b = finish_pair<COMB>(zk, zm, eb[0]);
+                    const float2 x2 = (COMB || k == 0) ? b.hi : b.lo;  // position M-k, before conj
+                    const uint64_t o = (f0 + g) * (uint64_t)(M + 1);
+                    if constexpr (MODE == OUT_POWER) {
+                        half_store(power_of(b.lo), out + o + k);
+                        half_store(power_of(x2), out + o + (M - k));
+                    } else {
+                        float2* o2 = reinterpret_cast<float2*>(out) + o;
+                        half_store(value_of(b.lo), o2 + k);
+                        half_store(k == 0 ? value_of(x2) : conj_value_of(x2), o2 + (M - k));
+                    }
+                }
+            }
+            const uint32_t gm = wv * FW + lane;
+            if (lane < FW && gm < nf) {
+                const float2* z = lds + gm * PADM;
+                const float2 zh2 = zat(z, M / 2, cmul(eb[1], eb[1]));  // W_M^(M/2) = -1 when folding
+                const BinPair b = finish_pair<COMB>(zh2, zh2, eb[1]);
+                const uint64_t o = (f0 + gm) * (uint64_t)(M + 1) + M / 2;
+                if constexpr (MODE == OUT_POWER)
+                    half_store(power_of(b.lo), out + o);
+                else
+                    half_store(value_of(b.lo), reinterpret_cast<float2*>(out) + o);
             }
         } else if constexpr (SYM_REAL) {
             // POWER / HALF_SPECTRUM: M+1 values per frame.  Slot k = 0 .. M/2 of
