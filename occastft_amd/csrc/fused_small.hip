@@ -58,7 +58,7 @@ struct BatchWaves {
     static constexpr int value = (LOG_M == 9 || LOG_M == 11 || LOG_M >= 13) ? 1 : 4;
 };
 
-// The N/2+1 layouts of the smallest window in per-frame slots (fused_frames):
+// The N/2+1 layouts of windows 64 and 128 in per-frame slots (fused_frames):
 // one pair of bins per slot and no division per value, where the flattened
 // form evaluated a pair per value.
 #ifndef OSTFT_SLOT_SMALL
@@ -288,9 +288,9 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
     constexpr bool SLOT_FRAME = SYM_REAL && T >= OSTFT_SLOT_MIN_T;
     constexpr bool SLOT_WAVE = SLOT_FRAME && T < 64;
     static_assert(!SLOT_WAVE || VPT * T >= 128, "a wave's 64 lanes take slots k < M/2 in whole runs");
-    // Below the symmetry threshold (the smallest window) the same slots, with
-    // 64 / (M/2) frames side by side in a wave (lane = frame x M/2 + slot)
-    constexpr bool SLOT_SMALL = OSTFT_SLOT_SMALL && !SYM_REAL && !DIRECT && M / 2 <= 64 && 64 % (M / 2) == 0 &&
+    // Below that (windows 64 and 128) the same slots, with 64 / (M/2) frames
+    // side by side in a wave (lane = frame x M/2 + slot)
+    constexpr bool SLOT_SMALL = OSTFT_SLOT_SMALL && !SLOT_FRAME && !DIRECT && M / 2 <= 64 && 64 % (M / 2) == 0 &&
                                 (64 * VPT / M) % (128 / M) == 0;
     // at the largest transforms (one frame per workgroup, four passes) the
     // slot twiddles are loaded after the FFT, which keeps their registers free
@@ -342,6 +342,10 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
                     eb[j] = twn[t + j * T];
                 eb[VPT / 2] = twn[M / 2];
             }
+        } else if constexpr (SLOT_SMALL) {
+            // the slot of a lane is the same in every frame it takes
+            eb[0] = twn[(tid & 63u) & (uint32_t)(M / 2 - 1)];
+            eb[1] = twn[M / 2];
         } else if constexpr (SYM_REAL) {
 #pragma unroll
             for (int it = 0; it < OITERS; ++it) {
@@ -349,10 +353,6 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
                 if (e < (uint32_t)(G * (M / 2 + 1)))
                     eb[it] = twn[e % (uint32_t)(M / 2 + 1)];
             }
-        } else if constexpr (SLOT_SMALL) {
-            // the slot of a lane is the same in every frame it takes
-            eb[0] = twn[(tid & 63u) & (uint32_t)(M / 2 - 1)];
-            eb[1] = twn[M / 2];
         } else {
 #pragma unroll
             for (int it = 0; it < OITERP; ++it) {
@@ -727,7 +727,7 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
                 }
             }
         } else if constexpr (SLOT_SMALL) {
-            // POWER / HALF_SPECTRUM at the smallest window, M+1 values per
+            // POWER / HALF_SPECTRUM at windows 64 and 128, M+1 values per
             // frame: a wave's FW frames, FS at a time, lane = frame within the
             // step x M/2 + slot k (slots as in the SLOT_WAVE branch); the
             // middle slots apart, one per lane
