@@ -768,6 +768,8 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
                     half_store(value_of(b.lo), reinterpret_cast<float2*>(out) + o);
             }
         } else if constexpr (SYM_REAL) {
+            // (window 128 built with OSTFT_SLOT_SMALL=0, the measured
+            // alternative to the slot form above)
             // POWER / HALF_SPECTRUM: M+1 values per frame.  Slot k = 0 .. M/2 of
             // a frame gives positions k and M-k from one pair (Z[k], Z[M-k]):
             // X[k] and X[M-k] = conj X[M+k] after the last radix-2 stage, E[k]
@@ -803,6 +805,8 @@ __device__ __forceinline__ void fused_frames(const FusedArgs& a, const uint64_t 
                 }
             }
         } else {
+            // (window 64 built with OSTFT_SLOT_SMALL=0, the measured
+            // alternative to the slot form above)
             // POWER / HALF_SPECTRUM at the smallest window: two consecutive
             // values of the flattened [frame][bin] index per thread and
             // iteration, one 8-byte (POWER) or 16-byte (HALF_SPECTRUM) store
