@@ -169,13 +169,14 @@ def test_batch_variant_matches_requests_run_alone(runner, shape, plan, form):
         assert bitwise_equal(o.cpu().numpy(), a), f"request {i}: exp {reqs[i][1].exp} out {reqs[i][1].output}"
 
 
-@pytest.mark.parametrize("exp,plan", [(6, 1), (8, 1), (9, 0), (10, 0), (10, 2), (11, 1), (8, 2), (11, 3),
-                                      (14, 2)])
+@pytest.mark.parametrize("exp,plan", [(6, 0), (6, 1), (7, 0), (7, 1), (8, 1), (9, 0), (10, 0), (10, 2), (11, 1),
+                                      (8, 2), (11, 3), (14, 2)])
 def test_plan_any_hop(runner, exp, plan):
     """A plan with an explicit geometry: hop 1, a hop past the window (fewer
     frames per tile than the workgroup holds), both N/2+1 layouts, against
-    the FP64 pipeline (windows 512 and 1024 write these layouts a wave per
-    frame run)."""
+    the FP64 pipeline (windows 256 to 2048 write these layouts a wave per
+    frame run, windows 64 and 128 with 4 and 2 frames side by side in a
+    wave)."""
     from test_mfma import _activate_geometry
 
     runner.engine_variant(exp, plan=plan)
