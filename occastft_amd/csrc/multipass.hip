@@ -99,6 +99,11 @@ void Scratch::destroy()
 // transposition writes one value per column across consecutive lanes, so an
 // odd column stride spreads those lanes over all banks (R + R/16 is a
 // multiple of 4: up to 16 lanes of a wave on one bank).
+// First pass: 16-byte loads of two neighbouring packed samples of a row
+// where the signal and the window allow it (half the load instructions).
+#ifndef OSTFT_MP_VEC4
+#define OSTFT_MP_VEC4 1
+#endif
 #ifndef OSTFT_MP_COL_PAD
 #define OSTFT_MP_COL_PAD 1
 #endif
@@ -156,7 +161,28 @@ __global__ void __launch_bounds__(MP_THREADS) first_pass_kernel(const PassArgs a
     const uint64_t fs = P * a.hop;
     const float2* win2 = reinterpret_cast<const float2*>(a.win);
     const uint64_t n_last = g0 + (CB - 1) + (uint64_t)(R - 1) * stride;  // the block's last packed index
-    if ((reinterpret_cast<uintptr_t>(a.sig + fs) & 7u) == 0 && fs + 2 * n_last + 2 <= a.L) {
+    const bool inside = fs + 2 * n_last + 2 <= a.L;
+    if (OSTFT_MP_VEC4 && CB >= 2 && inside && (reinterpret_cast<uintptr_t>(a.sig + fs) & 15u) == 0 &&
+        (reinterpret_cast<uintptr_t>(win2) & 15u) == 0) {
+        // pairs of neighbouring packed samples (columns c, c + 1 of a row;
+        // n even, so each pair is one 16-byte block of signal and of window)
+        constexpr int CH = CB >= 2 ? CB / 2 : 1;
+        float4 xv[8], wv[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const uint32_t e = tid + MP_THREADS * j;
+            const uint32_t n = g0 + 2u * (e % CH) + (e / CH) * stride;
+            xv[j] = *reinterpret_cast<const float4*>(a.sig + fs + 2ull * n);
+            wv[j] = win2 ? *reinterpret_cast<const float4*>(win2 + n) : make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const uint32_t e = tid + MP_THREADS * j;
+            const uint32_t c = 2u * (e % CH), m = e / CH;
+            lds[c * PADR + pad(m)] = make_float2(xv[j].x * wv[j].x, xv[j].y * wv[j].y);
+            lds[(c + 1) * PADR + pad(m)] = make_float2(xv[j].z * wv[j].z, xv[j].w * wv[j].w);
+        }
+    } else if ((reinterpret_cast<uintptr_t>(a.sig + fs) & 7u) == 0 && inside) {
         // the whole block lies inside the signal and its pairs are 8-byte
         // aligned: every load is issued before the first one is used
         float2 xv[16], wv[16];
