@@ -104,6 +104,10 @@ void Scratch::destroy()
 #ifndef OSTFT_MP_VEC4
 #define OSTFT_MP_VEC4 1
 #endif
+// First pass: its scratch rows written two values per 16-byte store.
+#ifndef OSTFT_MP_VEC4_ST
+#define OSTFT_MP_VEC4_ST 1
+#endif
 #ifndef OSTFT_MP_COL_PAD
 #define OSTFT_MP_COL_PAD 1
 #endif
@@ -225,11 +229,23 @@ __global__ void __launch_bounds__(MP_THREADS) first_pass_kernel(const PassArgs a
 
     // Y[g R + q]: each butterfly's R outputs are contiguous
     const uint64_t obase = (P - a.frame0) * M + ((uint64_t)g0 << LOG_R);
+    if constexpr (OSTFT_MP_VEC4_ST) {
+        // two neighbouring outputs of a butterfly per 16-byte store (obase
+        // and e even)
 #pragma unroll
-    for (int j = 0; j < 16; ++j) {
-        const uint32_t e = tid + MP_THREADS * j;
-        const uint32_t c = e >> LOG_R, q = e & (R - 1);
-        a.dst[obase + e] = lds[c * PADR + pad(q)];
+        for (int j = 0; j < 8; ++j) {
+            const uint32_t e = 2u * (tid + MP_THREADS * j);
+            const uint32_t c = e >> LOG_R, q = e & (R - 1);
+            const float2 v0 = lds[c * PADR + pad(q)], v1 = lds[c * PADR + pad(q + 1)];
+            *reinterpret_cast<float4*>(a.dst + obase + e) = make_float4(v0.x, v0.y, v1.x, v1.y);
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const uint32_t e = tid + MP_THREADS * j;
+            const uint32_t c = e >> LOG_R, q = e & (R - 1);
+            a.dst[obase + e] = lds[c * PADR + pad(q)];
+        }
     }
 }
 
